@@ -1,0 +1,166 @@
+/*
+ * pl_amd.h -- C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * This is the one boundary between host code and the HIP kernels.  The host
+ * C++ classes in lightning_unification_amd/csrc/host/ (StateVector,
+ * Measurements, Observables, AdjointJacobian -- the C++ API surface of
+ * Lightning-Qubit) and the pybind11 module `lightning_qubit_ops` sit on top of
+ * it; a ctypes / cgo / N-API binding can sit on it directly.
+ *
+ * Conventions
+ *  - Every function returns 0 on success.  On failure it returns a nonzero
+ *    status and pl_last_error() (thread-local) holds the message:
+ *      PL_ERR_INVALID  invalid argument    (reference: std::invalid_argument /
+ *                                           PL_ABORT -> LightningException,
+ *                                           core/src/utils/Error.hpp:22-128)
+ *      PL_ERR_RANGE    unknown name / id   (std::out_of_range from
+ *                                           unordered_map::at,
+ *                                           DynamicDispatcher.hpp:223-226)
+ *      PL_ERR_DEVICE   HIP runtime failure
+ *  - A state vector is an opaque handle to 2^n complex amplitudes resident in
+ *    HBM, interleaved (re, im), complex64 (PL_C64) or complex128 (PL_C128).
+ *    Wire w is bit (n-1-w) of the basis index, as in Lightning-Qubit
+ *    (GateImplementationsLM.hpp:209).
+ *  - Gate ids follow Pennylane::Gates::GateOperation
+ *    (core/src/gates/GateOperation.hpp:27-70): Identity=0 ... MultiRZ=33.
+ *    Generator calls take the id of the gate whose generator is applied.
+ *  - Parameters always cross as double and matrices as interleaved double
+ *    complex, row-major, wires[0] = most significant bit of the matrix index;
+ *    they are rounded to the state's precision on the device side.
+ *  - Tape encoding (OpsData, core/src/algorithms/JacobianData.hpp:40-200):
+ *      nops; ids[nops] (-1 = state preparation, skipped by the adjoint);
+ *      nwires[nops] + wires[]; inverse[nops]; nparams[nops] + params[].
+ *  - Observable encoding (Observable tree, core/src/observables/Observables.hpp):
+ *      int32 program [nobs, { is_ham, nterms, { nfactors,
+ *                      { kind, gate_id, nw, w_0 .. w_{nw-1} } } } ]
+ *      kind 0 = named gate, 1 = Hermitian matrix (4^nw complex taken in order
+ *      from `mats`); is_ham = 1 takes one coefficient per term, in order,
+ *      from `coeffs`.
+ *  - Calls on one handle are ordered on that handle's HIP stream; calls that
+ *    return host data synchronise.  Handles are not thread-safe.
+ */
+#ifndef PL_AMD_H
+#define PL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PL_C64 = 0, PL_C128 = 1 };
+enum { PL_OK = 0, PL_ERR_INVALID = 1, PL_ERR_RANGE = 2, PL_ERR_DEVICE = 3 };
+
+typedef struct pl_sv *pl_sv_t;
+
+/* Thread-local message of the last failure on this thread. */
+const char *pl_last_error(void);
+
+/* Device name, CU count, HBM size (replaces runtime_info / backend_info,
+ * core/src/bindings/Bindings.hpp:203-268, LQubitBindings.hpp:251-264). */
+int pl_device_info(char *buf, size_t len);
+
+/* ---- state lifecycle --------------------------------------------------- */
+
+/* New device state |0...0> (StateVectorLQubitManaged(num_qubits),
+ * simulators/lightning_qubit/StateVectorLQubitManaged.hpp:72-79). */
+int pl_sv_create(int num_qubits, int dtype, pl_sv_t *out);
+int pl_sv_destroy(pl_sv_t sv);
+int pl_sv_num_qubits(pl_sv_t sv, int *out);
+int pl_sv_dtype(pl_sv_t sv, int *out);
+/* Host -> device copy of 2^n amplitudes (StateVectorLQubitRaw(ptr, len) /
+ * updateData(ptr, len), StateVectorLQubitRaw.hpp:78-87,
+ * StateVectorLQubitManaged.hpp:166-169).  nbytes must equal the state size. */
+int pl_sv_upload(pl_sv_t sv, const void *host, size_t nbytes);
+/* Device -> host copy (getDataVector / reading the numpy buffer). */
+int pl_sv_download(pl_sv_t sv, void *host, size_t nbytes);
+/* dst <- src (copy constructor, StateVectorLQubitManaged.hpp:89-94). */
+int pl_sv_copy(pl_sv_t dst, pl_sv_t src);
+/* |0...0> in place. */
+int pl_sv_reset(pl_sv_t sv);
+/* Basis state |index> in place (BasisState preparation). */
+int pl_sv_set_basis_state(pl_sv_t sv, uint64_t index);
+/* Raw device pointer of the amplitudes (for interop; stays owned by sv). */
+int pl_sv_device_ptr(pl_sv_t sv, void **out);
+int pl_sv_synchronize(pl_sv_t sv);
+
+/* ---- gates --------------------------------------------------------------- */
+
+/* StateVectorLQubit::applyOperation(name, wires, inverse, params)
+ * (StateVectorLQubit.hpp:231-240; kernels GateImplementationsLM.hpp:389-1352). */
+int pl_sv_apply_gate(pl_sv_t sv, int gate_id, const int *wires, int nwires,
+                     int inverse, const double *params, int nparams);
+/* StateVectorBase::applyOperations (StateVectorBase.hpp:84-120): a whole tape
+ * in one call. */
+int pl_sv_apply_ops(pl_sv_t sv, int nops, const int *ids, const int *nwires,
+                    const int *wires, const uint8_t *inverse,
+                    const int *nparams, const double *params);
+/* StateVectorLQubit::applyMatrix(matrix, wires, inverse)
+ * (StateVectorLQubit.hpp:329-347; LM.hpp:204-325, PI.hpp:250-290). */
+int pl_sv_apply_matrix(pl_sv_t sv, const double *matrix, const int *wires,
+                       int nwires, int inverse);
+/* StateVectorLQubit::applyGenerator(name, wires, adj) -> scale
+ * (StateVectorLQubit.hpp:266-275; LM.hpp:1355-1640). */
+int pl_sv_apply_generator(pl_sv_t sv, int gate_id, const int *wires,
+                          int nwires, int adj, double *scale);
+
+/* ---- measurements ---------------------------------------------------------- */
+
+/* Measurements::expval(Observable) / var(Observable)
+ * (MeasurementsLQubit.hpp:248-295).  prog/coeffs/mats hold one observable. */
+int pl_expval(pl_sv_t sv, const int *prog, const double *coeffs,
+              const double *mats, double *out);
+int pl_var(pl_sv_t sv, const int *prog, const double *coeffs,
+           const double *mats, double *out);
+/* Measurements::expval/var(name, wires) (MeasurementsLQubit.hpp:167-181,298-317). */
+int pl_expval_named(pl_sv_t sv, int gate_id, const int *wires, int nwires,
+                    double *out);
+int pl_var_named(pl_sv_t sv, int gate_id, const int *wires, int nwires,
+                 double *out);
+/* Measurements::expval(matrix, wires) (MeasurementsLQubit.hpp:145-158). */
+int pl_expval_matrix(pl_sv_t sv, const double *matrix, const int *wires,
+                     int nwires, double *out);
+/* Measurements::probs() (nwires < 0, out has 2^n entries) and probs(wires)
+ * (out has 2^nwires entries, caller's wire order), MeasurementsLQubit.hpp:79-135. */
+int pl_probs(pl_sv_t sv, const int *wires, int nwires, double *out);
+/* Inverse-CDF sampling with caller-supplied uniforms in [0,1): idx[s] is the
+ * basis index with C_i < u_s <= C_{i+1} (C exclusive prefix of |psi|^2 in
+ * FP64, C_N := 1), the rule of lightning_kokkos
+ * MeasurementsKokkos.hpp:700-741 / MeasuresFunctors.hpp:55-105.  Replaces the
+ * unseedable alias sampler of MeasurementsLQubit.hpp:482-558. */
+int pl_sample(pl_sv_t sv, const double *uniforms, int64_t nsamples,
+              int64_t *idx);
+/* Same draws expanded to bits, samples[s*n + (n-1-j)] = (idx_s >> j) & 1
+ * (the layout of generate_samples, MeasurementsLQubit.hpp:546-552); the
+ * uniforms come from a SplitMix64 stream seeded by `seed`. */
+int pl_generate_samples(pl_sv_t sv, uint64_t seed, int64_t nsamples,
+                        uint64_t *bits);
+
+/* ---- adjoint Jacobian ------------------------------------------------------ */
+
+/* AdjointJacobian::adjointJacobian(jac, JacobianData{ops, obs, tp})
+ * (AdjointJacobianLQubit.hpp:212-336).  The sweep starts from the state held
+ * by sv (not modified).  jac is observable-major: jac[k*ntp + t]. */
+int pl_adjoint_jacobian(pl_sv_t sv, int nops, const int *ids,
+                        const int *nwires, const int *wires,
+                        const uint8_t *inverse, const int *nparams,
+                        const double *params, const int *prog,
+                        const double *coeffs, const double *mats, int ntp,
+                        const int *tp, double *jac);
+
+/* ---- timing ------------------------------------------------------------------ */
+
+/* HIP events recorded on the handle's stream around the calls in between;
+ * pl_timer_stop synchronises and returns the elapsed milliseconds. */
+int pl_timer_start(pl_sv_t sv);
+int pl_timer_stop(pl_sv_t sv, double *ms);
+/* Event slots (0 <= slot < 65536) recorded on the handle's stream without
+ * synchronising, for per-launch timing; pl_event_elapsed waits for slot b. */
+int pl_event_record(pl_sv_t sv, int slot);
+int pl_event_elapsed(pl_sv_t sv, int slot_a, int slot_b, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PL_AMD_H */

@@ -1,0 +1,250 @@
+"""ctypes binding of the C-ABI in include/pl_amd.h (libpl_amd.so).
+
+This is the thinnest host over the HIP engine: one Python object per device
+state vector, every call forwarded to the C entry point of the same name.
+It fails loudly when the shared library is missing; there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .circuits import encode_observables
+from .gates import GATE_ID
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpl_amd.so")
+
+PL_C64, PL_C128 = 0, 1
+_ERRORS = {1: ValueError, 2: IndexError, 3: RuntimeError}
+
+# every symbol declared in include/pl_amd.h
+EXPORTS = [
+    "pl_last_error", "pl_device_info", "pl_sv_create", "pl_sv_destroy", "pl_sv_num_qubits",
+    "pl_sv_dtype", "pl_sv_upload", "pl_sv_download", "pl_sv_copy", "pl_sv_reset",
+    "pl_sv_set_basis_state", "pl_sv_device_ptr", "pl_sv_synchronize", "pl_sv_apply_gate",
+    "pl_sv_apply_ops", "pl_sv_apply_matrix", "pl_sv_apply_generator", "pl_expval", "pl_var",
+    "pl_expval_named", "pl_var_named", "pl_expval_matrix", "pl_probs", "pl_sample",
+    "pl_generate_samples", "pl_adjoint_jacobian", "pl_timer_start", "pl_timer_stop",
+    "pl_event_record", "pl_event_elapsed",
+]
+
+_lib = None
+
+
+def load():
+    """Load libpl_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.pl_last_error.restype = ctypes.c_char_p
+        lib.pl_sv_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        for name in EXPORTS:
+            fn = getattr(lib, name)
+            if name != "pl_last_error":
+                fn.restype = ctypes.c_int
+        lib.pl_sv_set_basis_state.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+        lib.pl_sample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        lib.pl_generate_samples.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64,
+                                            ctypes.c_void_p]
+        lib.pl_sv_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        lib.pl_sv_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().pl_last_error().decode()
+        raise _ERRORS.get(rc, RuntimeError)(msg)
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(512)
+    _check(load().pl_device_info(buf, 512))
+    return buf.value.decode()
+
+
+def _i32(x):
+    a = np.ascontiguousarray(np.asarray(x, np.int32).reshape(-1))
+    return a if a.size else np.zeros(1, np.int32)
+
+
+def _f64(x):
+    a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+    return a if a.size else np.zeros(1, np.float64)
+
+
+def _cmat(m):
+    return np.ascontiguousarray(np.asarray(m, np.complex128).reshape(-1)).view(np.float64)
+
+
+class DeviceStateVector:
+    """A state vector resident in HBM (pl_sv_t)."""
+
+    def __init__(self, num_qubits, dtype=np.complex128):
+        lib = load()
+        self.dtype = np.dtype(dtype)
+        code = PL_C128 if self.dtype == np.complex128 else PL_C64
+        if self.dtype not in (np.complex128, np.complex64):
+            raise TypeError("dtype must be complex64 or complex128")
+        h = ctypes.c_void_p()
+        _check(lib.pl_sv_create(int(num_qubits), code, ctypes.byref(h)))
+        self.h = h
+        self.num_qubits = int(num_qubits)
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        n = int(arr.size).bit_length() - 1
+        if arr.ndim != 1 or arr.size != 1 << n:
+            raise ValueError("state must be a 1-D array of length 2^n")
+        sv = cls(n, arr.dtype)
+        sv.upload(arr)
+        return sv
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pl_sv_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def size(self):
+        return 1 << self.num_qubits
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        _check(load().pl_sv_upload(self.h, _p(arr), arr.nbytes))
+
+    def state(self):
+        out = np.empty(self.size, self.dtype)
+        _check(load().pl_sv_download(self.h, _p(out), out.nbytes))
+        return out
+
+    def copy_from(self, other):
+        _check(load().pl_sv_copy(self.h, other.h))
+
+    def reset(self):
+        _check(load().pl_sv_reset(self.h))
+
+    def set_basis_state(self, index):
+        _check(load().pl_sv_set_basis_state(self.h, int(index)))
+
+    def synchronize(self):
+        _check(load().pl_sv_synchronize(self.h))
+
+    # ---- gates ---------------------------------------------------------------
+    def apply(self, name, wires, inverse=False, params=()):
+        w, p = _i32(wires), _f64(params)
+        _check(load().pl_sv_apply_gate(self.h, GATE_ID[name], _p(w), len(wires), int(inverse),
+                                       _p(p), len(params)))
+
+    def apply_tape(self, tape):
+        ids, nw, wires, inv, npar, par = tape.encode()
+        _check(load().pl_sv_apply_ops(self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)),
+                                      _p(_i32(wires)), _p(inv if inv.size else np.zeros(1, np.uint8)),
+                                      _p(_i32(npar)), _p(_f64(par))))
+
+    def apply_matrix(self, matrix, wires, inverse=False):
+        m, w = _cmat(matrix), _i32(wires)
+        _check(load().pl_sv_apply_matrix(self.h, _p(m), _p(w), len(wires), int(inverse)))
+
+    def apply_generator(self, name, wires, adj=False):
+        w = _i32(wires)
+        s = ctypes.c_double()
+        _check(load().pl_sv_apply_generator(self.h, GATE_ID[name], _p(w), len(wires), int(adj),
+                                            ctypes.byref(s)))
+        return s.value
+
+    # ---- measurements ----------------------------------------------------------
+    def _obs(self, ob):
+        prog, coeffs, mats = encode_observables([ob])
+        return prog, coeffs, np.ascontiguousarray(mats).view(np.float64)
+
+    def expval(self, ob):
+        prog, coeffs, mats = self._obs(ob)
+        out = ctypes.c_double()
+        _check(load().pl_expval(self.h, _p(prog), _p(coeffs), _p(mats), ctypes.byref(out)))
+        return out.value
+
+    def var(self, ob):
+        prog, coeffs, mats = self._obs(ob)
+        out = ctypes.c_double()
+        _check(load().pl_var(self.h, _p(prog), _p(coeffs), _p(mats), ctypes.byref(out)))
+        return out.value
+
+    def expval_named(self, name, wires, var=False):
+        w = _i32(wires)
+        out = ctypes.c_double()
+        fn = load().pl_var_named if var else load().pl_expval_named
+        _check(fn(self.h, GATE_ID[name], _p(w), len(wires), ctypes.byref(out)))
+        return out.value
+
+    def expval_matrix(self, matrix, wires):
+        m, w = _cmat(matrix), _i32(wires)
+        out = ctypes.c_double()
+        _check(load().pl_expval_matrix(self.h, _p(m), _p(w), len(wires), ctypes.byref(out)))
+        return out.value
+
+    def probs(self, wires=None):
+        if wires is None:
+            out = np.empty(self.size, np.float64)
+            _check(load().pl_probs(self.h, _p(np.zeros(1, np.int32)), -1, _p(out)))
+        else:
+            out = np.empty(1 << len(wires), np.float64)
+            _check(load().pl_probs(self.h, _p(_i32(wires)), len(wires), _p(out)))
+        return out
+
+    def sample(self, uniforms):
+        u = _f64(uniforms)
+        idx = np.empty(max(1, len(uniforms)), np.int64)
+        _check(load().pl_sample(self.h, _p(u), len(uniforms), _p(idx)))
+        return idx[: len(uniforms)]
+
+    def generate_samples(self, num_samples, seed):
+        out = np.empty((max(1, num_samples), self.num_qubits), np.uint64)
+        _check(load().pl_generate_samples(self.h, int(seed) & ((1 << 64) - 1), int(num_samples),
+                                          _p(out)))
+        return out[:num_samples]
+
+    def adjoint_jacobian(self, tape, obs_list, tp):
+        ids, nw, wires, inv, npar, par = tape.encode()
+        prog, coeffs, mats = encode_observables(obs_list)
+        mats = np.ascontiguousarray(mats).view(np.float64)
+        jac = np.zeros(max(1, len(obs_list) * len(tp)), np.float64)
+        _check(load().pl_adjoint_jacobian(
+            self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+            _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)), _p(_f64(par)),
+            _p(prog), _p(coeffs), _p(mats), len(tp), _p(_i32(tp)), _p(jac)))
+        return jac[: len(obs_list) * len(tp)].reshape(len(obs_list), len(tp))
+
+    # ---- timing ----------------------------------------------------------------
+    def timer_start(self):
+        _check(load().pl_timer_start(self.h))
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        _check(load().pl_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
+    def event_record(self, slot):
+        _check(load().pl_event_record(self.h, int(slot)))
+
+    def event_elapsed(self, a, b):
+        ms = ctypes.c_double()
+        _check(load().pl_event_elapsed(self.h, int(a), int(b), ctypes.byref(ms)))
+        return ms.value

@@ -1,0 +1,190 @@
+// Fused adjoint-Jacobian step (one operation of the backward sweep of
+// AdjointJacobianLQubit.hpp:282-329).
+//
+// Reference order per op i:  mu <- lambda;  lambda <- U_i^dag lambda;
+// mu <- G_i mu;  J[t][k] = -2 s Im<H_k|mu>;  H_k <- U_i^dag H_k.
+// Here, per amplitude group of op i (U_i and G_i lowered on one geometry):
+// load lambda's group, form mu = G lambda in registers, store U^dag lambda;
+// then for each H_k: load its group, accumulate Im(conj(H_k) . mu), store
+// U^dag H_k.  mu never touches HBM and every state is read and written once
+// per op: (2 + 2K) state passes against the reference's (6 + 4K).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels_reduce.hpp"
+
+namespace plamd {
+
+template <class T, int M> struct AdjOp {
+    Geom geo;
+    uint64_t off[M];
+    dcx<T> ud[M * M];
+    dcx<T> g[M * M];
+    int ud_diag;
+};
+
+constexpr int kAdjMaxObs = 16; // observables per launch
+
+template <class T, int M>
+__device__ __forceinline__ void matvec(const dcx<T> *m, int diag, const v2<T> *x, v2<T> *y) {
+    if (diag) {
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            y[r] = cmul(m[r * M + r], x[r]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            v2<T> acc = cmul(m[r * M], x[0]);
+#pragma unroll
+            for (int c = 1; c < M; c++) {
+                cfma(acc, m[r * M + c], x[c]);
+            }
+            y[r] = acc;
+        }
+    }
+}
+
+// states[0] = lambda, states[1 .. nobs] = H_k of this launch.
+// TRAIN: accumulate Im<H_k|G lambda>; partial[block * nobs + k].
+// update_lambda: write U^dag lambda (only the last launch of an op does).
+template <class T, int M, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_adjoint_step(v2<T> *const *__restrict__ states,
+                                                         int nobs, uint64_t nwork,
+                                                         const AdjOp<T, M> op,
+                                                         int update_lambda,
+                                                         double *__restrict__ partial) {
+    double acc[kAdjMaxObs];
+#pragma unroll
+    for (int k = 0; k < kAdjMaxObs; k++) {
+        acc[k] = 0;
+    }
+    v2<T> *lam = states[0];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; g < nwork; g += stride) {
+        const uint64_t base = expand(g, op.geo);
+        v2<T> x[M], y[M], mu[M];
+        if (TRAIN || update_lambda) {
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                x[j] = lam[base | op.off[j]];
+            }
+        }
+        if (TRAIN) {
+            matvec<T, M>(op.g, 0, x, mu);
+        }
+        if (update_lambda) {
+            matvec<T, M>(op.ud, op.ud_diag, x, y);
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                lam[base | op.off[j]] = y[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kAdjMaxObs; k++) {
+            if (k < nobs) {
+                v2<T> *h = states[1 + k];
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    x[j] = h[base | op.off[j]];
+                }
+                if (TRAIN) {
+                    double s = 0;
+#pragma unroll
+                    for (int j = 0; j < M; j++) {
+                        // Im(conj(h) * mu)
+                        s += double(x[j].x) * double(mu[j].y) - double(x[j].y) * double(mu[j].x);
+                    }
+                    acc[k] += s;
+                }
+                matvec<T, M>(op.ud, op.ud_diag, x, y);
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    h[base | op.off[j]] = y[j];
+                }
+            }
+        }
+    }
+    if (TRAIN) {
+#pragma unroll
+        for (int k = 0; k < kAdjMaxObs; k++) {
+            if (k < nobs) { // nobs is uniform: every thread reaches the barrier
+                const double2 r = block_sum2(double2{acc[k], 0});
+                if (threadIdx.x == 0) {
+                    partial[static_cast<uint64_t>(blockIdx.x) * nobs + k] = r.x;
+                }
+            }
+        }
+    }
+}
+
+// Same step for parity-phase ops (MultiRZ): per amplitude, U^dag and G are
+// phases selected by the parity of i & mask.
+template <class T, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_adjoint_parity(v2<T> *const *__restrict__ states,
+                                                           int nobs, uint64_t len, uint64_t mask,
+                                                           dcx<T> u0, dcx<T> u1, dcx<T> g0,
+                                                           dcx<T> g1, int update_lambda,
+                                                           double *__restrict__ partial) {
+    double acc[kAdjMaxObs];
+#pragma unroll
+    for (int k = 0; k < kAdjMaxObs; k++) {
+        acc[k] = 0;
+    }
+    v2<T> *lam = states[0];
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        const bool odd = __popcll(i & mask) & 1;
+        const dcx<T> u = odd ? u1 : u0;
+        v2<T> x;
+        if (TRAIN || update_lambda) {
+            x = lam[i];
+        }
+        v2<T> mu;
+        if (TRAIN) {
+            mu = cmul(odd ? g1 : g0, x);
+        }
+        if (update_lambda) {
+            lam[i] = cmul(u, x);
+        }
+#pragma unroll
+        for (int k = 0; k < kAdjMaxObs; k++) {
+            if (k < nobs) {
+                v2<T> *h = states[1 + k];
+                const v2<T> hv = h[i];
+                if (TRAIN) {
+                    acc[k] += double(hv.x) * double(mu.y) - double(hv.y) * double(mu.x);
+                }
+                h[i] = cmul(u, hv);
+            }
+        }
+    }
+    if (TRAIN) {
+#pragma unroll
+        for (int k = 0; k < kAdjMaxObs; k++) {
+            if (k < nobs) { // nobs is uniform: every thread reaches the barrier
+                const double2 r = block_sum2(double2{acc[k], 0});
+                if (threadIdx.x == 0) {
+                    partial[static_cast<uint64_t>(blockIdx.x) * nobs + k] = r.x;
+                }
+            }
+        }
+    }
+}
+
+// jac[row + k] = factor * sum_b partial[b * nobs + k]   (one thread per k)
+__global__ void k_adjoint_final(const double *__restrict__ partial, int nblocks, int nobs,
+                                double factor, double *__restrict__ jac) {
+    const int k = threadIdx.x;
+    if (k < nobs) {
+        double s = 0;
+        for (int b = 0; b < nblocks; b++) {
+            s += partial[b * nobs + k];
+        }
+        jac[k] = factor * s;
+    }
+}
+
+} // namespace plamd

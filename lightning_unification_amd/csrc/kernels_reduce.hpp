@@ -1,0 +1,274 @@
+// Reduction kernels: inner products, Pauli-string expectation values, norms,
+// probabilities and marginals.  All sums accumulate in FP64 in a fixed order
+// (per-thread strided sums, wavefront shuffles, one LDS step, per-block
+// partials, one final block), so results are run-to-run reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels_gate.hpp"
+
+namespace plamd {
+
+constexpr int kReduceBlocks = 1024; // max partials per reduction
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x += __shfl_xor(x, o, 64);
+    }
+    return x;
+}
+
+// Sum of two doubles over a 256-thread block; result valid in thread 0.
+__device__ __forceinline__ double2 block_sum2(double2 v) {
+    __shared__ double sx[kBlock / 64], sy[kBlock / 64];
+    v.x = wave_sum(v.x);
+    v.y = wave_sum(v.y);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sx[wid] = v.x;
+        sy[wid] = v.y;
+    }
+    __syncthreads();
+    double2 r{0, 0};
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kBlock / 64; w++) {
+            r.x += sx[w];
+            r.y += sy[w];
+        }
+    }
+    __syncthreads();
+    return r;
+}
+
+// partial[b] = sum over the block's elements of conj(a_i) * b_i
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot(const v2<T> *__restrict__ a,
+                                                const v2<T> *__restrict__ b, uint64_t len,
+                                                double2 *__restrict__ partial) {
+    double2 acc{0, 0};
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        const v2<T> x = a[i], y = b[i];
+        acc.x += double(x.x) * double(y.x) + double(x.y) * double(y.y);
+        acc.y += double(x.x) * double(y.y) - double(x.y) * double(y.x);
+    }
+    const double2 r = block_sum2(acc);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = r;
+    }
+}
+
+// Sum `count` partials (fixed order) into out[0]; one block.
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const double2 *__restrict__ partial,
+                                                         int count, double2 *__restrict__ out) {
+    double2 acc{0, 0};
+    for (int i = threadIdx.x; i < count; i += kBlock) {
+        acc.x += partial[i].x;
+        acc.y += partial[i].y;
+    }
+    const double2 r = block_sum2(acc);
+    if (threadIdx.x == 0) {
+        out[0] = r;
+    }
+}
+
+// One Pauli word: x-mask (X and Y letters), zy-mask (Z and Y letters) and
+// the number of Y letters mod 4.  Its matrix element is
+//   <i|P|i^x> = (-i)^ny * (-1)^popcount(i & zy).
+struct PauliTerm {
+    uint64_t zy;
+    double coeff;
+    int ny;
+    int pad;
+};
+
+// partial[b] = sum_t coeff_t * Re<psi|P_t|psi> for terms sharing one x-mask.
+// x == 0: one pass over amplitudes.  x != 0: one pass over pairs (i, i^x)
+// with the lowest x bit of i clear, each amplitude read once.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pauli_expval(const v2<T> *__restrict__ sv,
+                                                         uint64_t nwork, uint64_t x, int lowbit,
+                                                         int nterms,
+                                                         const PauliTerm *__restrict__ terms,
+                                                         double2 *__restrict__ partial) {
+    double acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; g < nwork; g += stride) {
+        if (x == 0) {
+            const v2<T> a = sv[g];
+            const double p = double(a.x) * double(a.x) + double(a.y) * double(a.y);
+            double s = 0;
+            for (int t = 0; t < nterms; t++) {
+                const double sg = (__popcll(g & terms[t].zy) & 1) ? -1.0 : 1.0;
+                s += terms[t].coeff * sg;
+            }
+            acc += p * s;
+        } else {
+            const uint64_t lo = g & ((uint64_t{1} << lowbit) - 1);
+            const uint64_t i = ((g >> lowbit) << (lowbit + 1)) | lo;
+            const uint64_t j = i ^ x;
+            const v2<T> a = sv[i], c = sv[j];
+            // z = conj(a) * c
+            const double zr = double(a.x) * double(c.x) + double(a.y) * double(c.y);
+            const double zi = double(a.x) * double(c.y) - double(a.y) * double(c.x);
+            double s = 0;
+            for (int t = 0; t < nterms; t++) {
+                const double si = (__popcll(i & terms[t].zy) & 1) ? -1.0 : 1.0;
+                const double sj = (__popcll(j & terms[t].zy) & 1) ? -1.0 : 1.0;
+                // Re[(-i)^ny (si z + sj conj z)]
+                double v;
+                switch (terms[t].ny & 3) {
+                case 0: v = (si + sj) * zr; break;
+                case 1: v = (si - sj) * zi; break;
+                case 2: v = -(si + sj) * zr; break;
+                default: v = -(si - sj) * zi; break;
+                }
+                s += terms[t].coeff * v;
+            }
+            acc += s;
+        }
+    }
+    const double2 r = block_sum2(double2{acc, 0});
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = r;
+    }
+}
+
+// dst[i] (+)= c * (-i)^ny * (-1)^popcount(i & zy) * src[i ^ x]
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_apply_pauli(const v2<T> *__restrict__ src,
+                                                        v2<T> *__restrict__ dst, uint64_t len,
+                                                        uint64_t x, uint64_t zy, dcx<T> c,
+                                                        int accumulate) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        const v2<T> s = src[i ^ x];
+        dcx<T> f = c;
+        if (__popcll(i & zy) & 1) {
+            f.r = -f.r;
+            f.i = -f.i;
+        }
+        const v2<T> t = cmul(f, s);
+        if (accumulate) {
+            v2<T> d = dst[i];
+            d.x += t.x;
+            d.y += t.y;
+            dst[i] = d;
+        } else {
+            dst[i] = t;
+        }
+    }
+}
+
+// y[i] += c * x[i]   (scaleAndAdd, LinearAlgebra.hpp:776-839)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_axpy(const v2<T> *__restrict__ x, v2<T> *__restrict__ y,
+                                                 uint64_t len, dcx<T> c) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        v2<T> d = y[i];
+        const v2<T> t = cmul(c, x[i]);
+        d.x += t.x;
+        d.y += t.y;
+        y[i] = d;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_probs(const v2<T> *__restrict__ sv, uint64_t len,
+                                                  double *__restrict__ out) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        const v2<T> a = sv[i];
+        out[i] = double(a.x * a.x + a.y * a.y);
+    }
+}
+
+struct MarginalGeom {
+    int nw;
+    int bitpos[32];   // bit position of caller wire j (MSB of the bin index first)
+    Geom rest;        // pins the marginal wires (ascending), no fixed bits; nw <= 12
+    uint64_t free_mask; // bits not among the marginal wires
+};
+
+// Scatter the bits of r into the set bits of mask, lowest first (pdep).
+__device__ __forceinline__ uint64_t deposit(uint64_t r, uint64_t mask) {
+    uint64_t out = 0;
+    while (mask) {
+        const uint64_t low = mask & (~mask + 1);
+        if (r & 1U) {
+            out |= low;
+        }
+        r >>= 1;
+        mask ^= low;
+    }
+    return out;
+}
+
+__device__ __forceinline__ uint64_t bin_bits(uint64_t bin, const MarginalGeom &m) {
+    uint64_t r = 0;
+    for (int j = 0; j < m.nw; j++) {
+        if ((bin >> (m.nw - 1 - j)) & 1U) {
+            r |= uint64_t{1} << m.bitpos[j];
+        }
+    }
+    return r;
+}
+
+// Marginal probabilities, bins in the caller's wire order.  Grid: blockIdx.y
+// = bin, blockIdx.x = chunk of the remaining index space; partial[bin][chunk].
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_marginal(const v2<T> *__restrict__ sv,
+                                                     uint64_t nrest, MarginalGeom m,
+                                                     double *__restrict__ partial) {
+    const uint64_t bin = blockIdx.y;
+    const uint64_t bb = bin_bits(bin, m);
+    double acc = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; r < nrest; r += stride) {
+        const v2<T> a = sv[expand(r, m.rest) | bb];
+        acc += double(a.x * a.x + a.y * a.y);
+    }
+    const double2 s = block_sum2(double2{acc, 0});
+    if (threadIdx.x == 0) {
+        partial[bin * gridDim.x + blockIdx.x] = s.x;
+    }
+}
+
+// Final per-bin sum over `chunks` partials (thread per bin).
+__global__ __launch_bounds__(kBlock) void k_marginal_final(const double *__restrict__ partial,
+                                                           uint64_t nbins, int chunks,
+                                                           double *__restrict__ out) {
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (b < nbins) {
+        double s = 0;
+        for (int c = 0; c < chunks; c++) {
+            s += partial[b * chunks + c];
+        }
+        out[b] = s;
+    }
+}
+
+// Marginal when few amplitudes fall in each bin: thread per bin, sequential.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_marginal_small(const v2<T> *__restrict__ sv,
+                                                           uint64_t nbins, uint64_t nrest,
+                                                           MarginalGeom m,
+                                                           double *__restrict__ out) {
+    const uint64_t bin = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (bin >= nbins) {
+        return;
+    }
+    const uint64_t bb = bin_bits(bin, m);
+    double s = 0;
+    for (uint64_t r = 0; r < nrest; r++) {
+        const v2<T> a = sv[deposit(r, m.free_mask) | bb];
+        s += double(a.x * a.x + a.y * a.y);
+    }
+    out[bin] = s;
+}
+
+} // namespace plamd

@@ -1,0 +1,1274 @@
+// C-ABI of the gfx950 state-vector engine (declared in include/pl_amd.h).
+//
+// Host orchestration: every entry point validates its arguments, lowers the
+// request (lower.hpp) and enqueues kernels (kernels_*.hpp) on the handle's
+// HIP stream.  There is no CPU compute path: if the device work cannot be
+// enqueued the call fails with PL_ERR_DEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pl_amd.h"
+#include "kernels_adjoint.hpp"
+#include "kernels_gate.hpp"
+#include "kernels_reduce.hpp"
+#include "kernels_sample.hpp"
+#include "lower.hpp"
+
+namespace plamd {
+
+thread_local std::string g_err;
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));          \
+        }                                                                                  \
+    } while (0)
+
+int device_cus() {
+    static int cus = [] {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
+            return 256;
+        }
+        return p.multiProcessorCount;
+    }();
+    return cus;
+}
+
+// Grid for a grid-stride streaming kernel over `work` items, `per` items per
+// thread per iteration: enough workgroups to fill every CU several times,
+// but no more than the work needs.
+unsigned grid_for(uint64_t work, int per = 1, unsigned cap_per_cu = 32) {
+    const uint64_t need = (work + uint64_t(kBlock) * per - 1) / (uint64_t(kBlock) * per);
+    const uint64_t cap = uint64_t(device_cus()) * cap_per_cu;
+    return static_cast<unsigned>(std::max<uint64_t>(1, std::min(need, cap)));
+}
+
+// ---------------------------------------------------------------------------
+// Observable tree parsed from the program encoding of pl_amd.h
+// ---------------------------------------------------------------------------
+struct ObsNode {
+    int kind = 0; // 0 named, 1 hermitian, 2 tensor, 3 hamiltonian
+    int op = 0;
+    std::vector<int> wires;
+    std::vector<std::complex<double>> mat;
+    std::vector<double> coeffs;
+    std::vector<ObsNode> kids;
+};
+
+struct ObsReader {
+    const int *prog;
+    const double *coeffs;
+    const double *mats;
+    ObsNode factor() {
+        ObsNode o;
+        o.kind = *prog++;
+        o.op = *prog++;
+        const int nw = *prog++;
+        if (nw < 0 || nw > 30) {
+            throw InvalidArg("bad observable wire count");
+        }
+        o.wires.assign(prog, prog + nw);
+        prog += nw;
+        if (o.kind == 1) {
+            const size_t dim = size_t{1} << (2 * nw);
+            for (size_t i = 0; i < dim; i++) {
+                o.mat.emplace_back(mats[2 * i], mats[2 * i + 1]);
+            }
+            mats += 2 * dim;
+        } else if (o.kind == 0) {
+            gate_name(o.op); // range check
+        } else {
+            throw InvalidArg("bad observable factor kind");
+        }
+        return o;
+    }
+    ObsNode term() {
+        const int nf = *prog++;
+        if (nf < 1) {
+            throw InvalidArg("empty observable term");
+        }
+        if (nf == 1) {
+            return factor();
+        }
+        ObsNode t;
+        t.kind = 2;
+        std::vector<int> seen;
+        for (int f = 0; f < nf; f++) {
+            t.kids.push_back(factor());
+            for (int w : t.kids.back().wires) {
+                if (std::find(seen.begin(), seen.end(), w) != seen.end()) {
+                    // TensorProdObsBase ctor, Observables.hpp:238-246
+                    throw InvalidArg("All wires in observables must be disjoint.");
+                }
+                seen.push_back(w);
+            }
+        }
+        return t;
+    }
+    ObsNode one() {
+        const int is_ham = *prog++;
+        const int nt = *prog++;
+        if (!is_ham) {
+            if (nt != 1) {
+                throw InvalidArg("non-Hamiltonian observable must have one term");
+            }
+            return term();
+        }
+        ObsNode h;
+        h.kind = 3;
+        for (int t = 0; t < nt; t++) {
+            h.coeffs.push_back(*coeffs++);
+            h.kids.push_back(term());
+        }
+        return h;
+    }
+};
+
+// A Pauli word: (x-mask, zy-mask, #Y) -- the fast path of expval / var /
+// the adjoint's observable application.
+struct PauliWord {
+    uint64_t x = 0, zy = 0;
+    int ny = 0;
+};
+
+bool as_pauli(const ObsNode &o, int n, PauliWord &pw) {
+    auto letter = [&](const ObsNode &f) {
+        if (f.kind != 0 || f.wires.size() != 1) {
+            return false;
+        }
+        if (f.wires[0] < 0 || f.wires[0] >= n) {
+            throw InvalidArg("wire index out of range");
+        }
+        const uint64_t b = uint64_t{1} << (n - 1 - f.wires[0]);
+        switch (f.op) {
+        case Identity: return true;
+        case PauliX: pw.x |= b; return true;
+        case PauliY: pw.x |= b; pw.zy |= b; pw.ny++; return true;
+        case PauliZ: pw.zy |= b; return true;
+        default: return false;
+        }
+    };
+    pw = PauliWord{};
+    if (o.kind == 0) {
+        return letter(o);
+    }
+    if (o.kind == 2) {
+        for (const auto &k : o.kids) {
+            if (!letter(k)) {
+                return false;
+            }
+        }
+        return true;
+    }
+    return false;
+}
+
+} // namespace plamd
+
+using namespace plamd;
+
+struct pl_sv {
+    int n = 0;
+    int dtype = PL_C128;
+    uint64_t len = 0;
+    void *d = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> pool;   // scratch states (state-sized)
+    size_t pool_used = 0;
+    double *red = nullptr;      // reduction scratch (partials)
+    size_t red_bytes = 0;
+    void *staging = nullptr;    // device staging for matrices / small tables
+    size_t staging_bytes = 0;
+    std::vector<hipEvent_t> slots; // pl_event_record slots
+
+    size_t state_bytes() const { return len * (dtype == PL_C128 ? 16 : 8); }
+};
+
+namespace plamd {
+
+void *scratch_acquire(pl_sv *h) {
+    if (h->pool_used == h->pool.size()) {
+        void *p = nullptr;
+        HIP_OK(hipMalloc(&p, h->state_bytes()));
+        h->pool.push_back(p);
+    }
+    return h->pool[h->pool_used++];
+}
+struct ScratchScope {
+    pl_sv *h;
+    size_t mark;
+    explicit ScratchScope(pl_sv *hh) : h(hh), mark(hh->pool_used) {}
+    void *get() { return scratch_acquire(h); }
+    ~ScratchScope() { h->pool_used = mark; }
+};
+
+double *reduce_buf(pl_sv *h, size_t bytes) {
+    if (h->red_bytes < bytes) {
+        if (h->red) {
+            HIP_OK(hipStreamSynchronize(h->stream));
+            HIP_OK(hipFree(h->red));
+            h->red = nullptr;
+        }
+        HIP_OK(hipMalloc(reinterpret_cast<void **>(&h->red), bytes));
+        h->red_bytes = bytes;
+    }
+    return h->red;
+}
+
+// Copy host bytes to the handle's device staging area and wait for the copy,
+// so the host buffer may be released as soon as this returns.
+void *stage(pl_sv *h, const void *src, size_t bytes) {
+    if (h->staging_bytes < bytes) {
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (h->staging) {
+            HIP_OK(hipFree(h->staging));
+            h->staging = nullptr;
+        }
+        const size_t sz = std::max<size_t>(bytes, 1 << 16);
+        HIP_OK(hipMalloc(&h->staging, sz));
+        h->staging_bytes = sz;
+    }
+    HIP_OK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return h->staging;
+}
+
+template <class T> struct Engine {
+    using V = v2<T>;
+    pl_sv *h;
+    explicit Engine(pl_sv *hh) : h(hh) {}
+    V *psi() const { return static_cast<V *>(h->d); }
+    uint64_t len() const { return h->len; }
+    int n() const { return h->n; }
+
+    template <int M> void launch_small(const GroupOp<T> &g, V *sv) {
+        SmallOp<T, M> op{};
+        op.geo.fixed = g.fixed;
+        op.geo.nb = g.nb;
+        for (int i = 0; i < g.nb; i++) {
+            op.geo.pos[i] = g.pos[i];
+        }
+        for (int j = 0; j < M; j++) {
+            op.off[j] = g.off[j];
+        }
+        for (int j = 0; j < M * M; j++) {
+            op.mat[j] = {g.mat[j].real(), g.mat[j].imag()};
+        }
+        op.diag = g.diag ? 1 : 0;
+        const uint64_t nwork = len() >> g.nb;
+        constexpr int U = M <= 2 ? 2 : 1;
+        hipLaunchKernelGGL((k_group_small<T, M, U>), dim3(grid_for(nwork, U)), dim3(kBlock), 0,
+                           h->stream, sv, nwork, op);
+        HIP_OK(hipGetLastError());
+    }
+
+    template <int M> void launch_big(const GroupOp<T> &g, V *sv) {
+        BigOp<T, M> op{};
+        op.geo.fixed = g.fixed;
+        op.geo.nb = g.nb;
+        for (int i = 0; i < g.nb; i++) {
+            op.geo.pos[i] = g.pos[i];
+        }
+        for (int j = 0; j < M; j++) {
+            op.off[j] = g.off[j];
+        }
+        std::vector<dcx<T>> m(M * M);
+        for (int j = 0; j < M * M; j++) {
+            m[j] = {g.mat[j].real(), g.mat[j].imag()};
+        }
+        op.mat = static_cast<const dcx<T> *>(stage(h, m.data(), m.size() * sizeof(dcx<T>)));
+        const uint64_t nwork = len() >> g.nb;
+        hipLaunchKernelGGL((k_group_big<T, M>), dim3(grid_for(nwork)), dim3(kBlock), 0, h->stream,
+                           sv, nwork, op);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(h->stream)); // staging is reused by the next call
+    }
+
+    void launch_huge(const GroupOp<T> &g, V *sv) {
+        const size_t M = g.off.size();
+        std::vector<unsigned char> blob(M * 8 + M * M * sizeof(dcx<T>));
+        std::memcpy(blob.data(), g.off.data(), M * 8);
+        auto *m = reinterpret_cast<dcx<T> *>(blob.data() + M * 8);
+        for (size_t j = 0; j < M * M; j++) {
+            m[j] = {g.mat[j].real(), g.mat[j].imag()};
+        }
+        auto *dev = static_cast<unsigned char *>(stage(h, blob.data(), blob.size()));
+        Geom geo{};
+        geo.fixed = g.fixed;
+        geo.nb = g.nb;
+        for (int i = 0; i < g.nb; i++) {
+            geo.pos[i] = g.pos[i];
+        }
+        const uint64_t nwork = len() >> g.nb;
+        const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(nwork, device_cus() * 8));
+        hipLaunchKernelGGL((k_group_huge<T>), dim3(grid), dim3(kBlock), M * sizeof(V), h->stream,
+                           sv, nwork, geo, static_cast<int>(M),
+                           reinterpret_cast<const uint64_t *>(dev),
+                           reinterpret_cast<const dcx<T> *>(dev + M * 8));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+
+    void apply(const GroupOp<T> &g, V *sv) {
+        if (g.kind == GroupOp<T>::Noop) {
+            return;
+        }
+        if (g.kind == GroupOp<T>::Parity) {
+            hipLaunchKernelGGL((k_parity<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream, sv,
+                               len(), g.parity_mask,
+                               dcx<T>{g.parity_phase[0].real(), g.parity_phase[0].imag()},
+                               dcx<T>{g.parity_phase[1].real(), g.parity_phase[1].imag()});
+            HIP_OK(hipGetLastError());
+            return;
+        }
+        switch (g.M) {
+        case 1: return launch_small<1>(g, sv);
+        case 2: return launch_small<2>(g, sv);
+        case 4: return launch_small<4>(g, sv);
+        case 8: return launch_big<8>(g, sv);
+        case 16: return launch_big<16>(g, sv);
+        case 32: return launch_big<32>(g, sv);
+        default: return launch_huge(g, sv);
+        }
+    }
+
+    std::vector<T> params_t(const double *p, int np) const {
+        std::vector<T> v(np);
+        for (int i = 0; i < np; i++) {
+            v[i] = static_cast<T>(p[i]);
+        }
+        return v;
+    }
+
+    void gate(V *sv, int id, const int *w, int nw, bool inv, const double *p, int np) {
+        const auto pt = params_t(p, np);
+        apply(lower_gate<T>(n(), id, w, nw, inv, pt.data(), np), sv);
+    }
+
+    void matrix(V *sv, const double *m, const int *w, int nw, bool inv) {
+        if (nw <= 0) {
+            throw InvalidArg("Number of wires must be larger than 0");
+        }
+        if (nw > kMaxPos) {
+            throw InvalidArg("matrices on more than 12 wires are not supported");
+        }
+        const size_t dim = size_t{1} << (2 * nw);
+        std::vector<std::complex<T>> mt(dim);
+        for (size_t i = 0; i < dim; i++) {
+            mt[i] = {static_cast<T>(m[2 * i]), static_cast<T>(m[2 * i + 1])};
+        }
+        apply(lower_matrix<T>(n(), mt.data(), w, nw, inv), sv);
+    }
+
+    double generator(V *sv, int id, const int *w, int nw) {
+        T s = 0;
+        const auto g = lower_generator<T>(n(), id, w, nw, false, &s);
+        apply(g, sv);
+        return static_cast<double>(s);
+    }
+
+    // ---- reductions ----------------------------------------------------------
+    std::complex<double> dot(const V *a, const V *b) {
+        const unsigned grid = std::min<unsigned>(grid_for(len(), 1, 4), kReduceBlocks);
+        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + 1) * sizeof(double2)));
+        hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(kBlock), 0, h->stream, a, b, len(), part);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, (int)grid,
+                           part + kReduceBlocks);
+        HIP_OK(hipGetLastError());
+        double2 r;
+        HIP_OK(hipMemcpyAsync(&r, part + kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return {r.x, r.y};
+    }
+
+    // sum_t c_t <psi|P_t|psi> over Pauli words, grouped by x-mask
+    double pauli_expval(const V *sv, const std::vector<PauliWord> &words,
+                        const std::vector<double> &coeffs) {
+        std::map<uint64_t, std::vector<PauliTerm>> groups;
+        for (size_t t = 0; t < words.size(); t++) {
+            groups[words[t].x].push_back(PauliTerm{words[t].zy, coeffs[t], words[t].ny & 3, 0});
+        }
+        std::vector<PauliTerm> table;
+        std::vector<size_t> start;
+        for (auto &kv : groups) {
+            start.push_back(table.size());
+            table.insert(table.end(), kv.second.begin(), kv.second.end());
+        }
+        auto *dterms = static_cast<PauliTerm *>(stage(h, table.data(), table.size() * sizeof(PauliTerm)));
+        const size_t ng = groups.size();
+        auto *part = reinterpret_cast<double2 *>(
+            reduce_buf(h, (ng * kReduceBlocks + 1) * sizeof(double2)));
+        size_t gi = 0;
+        int total = 0;
+        std::vector<int> grids;
+        for (auto &kv : groups) {
+            const uint64_t x = kv.first;
+            const uint64_t nwork = x ? len() / 2 : len();
+            const int lowbit = x ? __builtin_ctzll(x) : 0;
+            const unsigned grid = std::min<unsigned>(grid_for(nwork, 1, 4), kReduceBlocks);
+            hipLaunchKernelGGL((k_pauli_expval<T>), dim3(grid), dim3(kBlock), 0, h->stream, sv,
+                               nwork, x, lowbit, static_cast<int>(kv.second.size()),
+                               dterms + start[gi], part + total);
+            HIP_OK(hipGetLastError());
+            total += grid;
+            gi++;
+        }
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, total,
+                           part + ng * kReduceBlocks);
+        HIP_OK(hipGetLastError());
+        double2 r;
+        HIP_OK(hipMemcpyAsync(&r, part + ng * kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return r.x;
+    }
+
+    // dst = P src (accumulate: dst += c P src)
+    void apply_pauli(const V *src, V *dst, const PauliWord &pw, std::complex<double> c, bool acc) {
+        // (-i)^ny
+        static const std::complex<double> ph[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
+        const std::complex<double> f = c * ph[pw.ny & 3];
+        hipLaunchKernelGGL((k_apply_pauli<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream,
+                           src, dst, len(), pw.x, pw.zy,
+                           dcx<T>{static_cast<T>(f.real()), static_cast<T>(f.imag())},
+                           acc ? 1 : 0);
+        HIP_OK(hipGetLastError());
+    }
+
+    void copy(V *dst, const V *src) {
+        HIP_OK(hipMemcpyAsync(dst, src, h->state_bytes(), hipMemcpyDeviceToDevice, h->stream));
+    }
+
+    // Observable::applyInPlace (Observables.hpp:98-324, ObservablesLQubit.hpp:156-355)
+    void apply_obs(V *sv, const ObsNode &o) {
+        switch (o.kind) {
+        case 0:
+            gate(sv, o.op, o.wires.data(), static_cast<int>(o.wires.size()), false, nullptr, 0);
+            return;
+        case 1: {
+            std::vector<double> m(o.mat.size() * 2);
+            for (size_t i = 0; i < o.mat.size(); i++) {
+                m[2 * i] = o.mat[i].real();
+                m[2 * i + 1] = o.mat[i].imag();
+            }
+            matrix(sv, m.data(), o.wires.data(), static_cast<int>(o.wires.size()), false);
+            return;
+        }
+        case 2:
+            for (const auto &k : o.kids) {
+                apply_obs(sv, k);
+            }
+            return;
+        default: {
+            // Hamiltonian: res = sum_t c_t (O_t psi), then psi <- res
+            ScratchScope sc(h);
+            V *res = static_cast<V *>(sc.get());
+            bool first = true;
+            V *tmp = nullptr;
+            for (size_t t = 0; t < o.kids.size(); t++) {
+                PauliWord pw;
+                if (as_pauli(o.kids[t], n(), pw)) {
+                    apply_pauli(sv, res, pw, {o.coeffs[t], 0}, !first);
+                } else {
+                    if (!tmp) {
+                        tmp = static_cast<V *>(sc.get());
+                    }
+                    copy(tmp, sv);
+                    apply_obs(tmp, o.kids[t]);
+                    if (first) {
+                        HIP_OK(hipMemsetAsync(res, 0, h->state_bytes(), h->stream));
+                    }
+                    hipLaunchKernelGGL((k_axpy<T>), dim3(grid_for(len())), dim3(kBlock), 0,
+                                       h->stream, tmp, res, len(),
+                                       dcx<T>{static_cast<T>(o.coeffs[t]), T(0)});
+                    HIP_OK(hipGetLastError());
+                }
+                first = false;
+            }
+            if (first) {
+                HIP_OK(hipMemsetAsync(res, 0, h->state_bytes(), h->stream));
+            }
+            copy(sv, res);
+        }
+        }
+    }
+
+    // out-of-place O psi into dst
+    void obs_into(V *dst, const V *src, const ObsNode &o) {
+        PauliWord pw;
+        if (as_pauli(o, n(), pw)) {
+            apply_pauli(src, dst, pw, {1, 0}, false);
+            return;
+        }
+        if (o.kind == 3) {
+            bool all = true;
+            std::vector<PauliWord> ws(o.kids.size());
+            for (size_t t = 0; t < o.kids.size() && all; t++) {
+                all = as_pauli(o.kids[t], n(), ws[t]);
+            }
+            if (all && !o.kids.empty()) {
+                for (size_t t = 0; t < ws.size(); t++) {
+                    apply_pauli(src, dst, ws[t], {o.coeffs[t], 0}, t > 0);
+                }
+                return;
+            }
+        }
+        copy(dst, src);
+        apply_obs(dst, o);
+    }
+
+    // Measurements::probs() / probs(wires) (MeasurementsLQubit.hpp:79-135): the
+    // marginal is computed directly in the caller's wire order, which equals
+    // the reference's sorted marginal followed by transpose_state_tensor.
+    void probs(const int *wires, int nw, double *out) {
+        if (nw < 0) {
+            ScratchScope sc(h); // a state buffer holds >= 2^n doubles
+            auto *d = static_cast<double *>(sc.get());
+            hipLaunchKernelGGL((k_probs<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream,
+                               psi(), len(), d);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(out, d, len() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+            HIP_OK(hipStreamSynchronize(h->stream));
+            return;
+        }
+        if (nw > n() || nw > 32) {
+            throw InvalidArg("too many wires for probs");
+        }
+        MarginalGeom m{};
+        m.nw = nw;
+        uint64_t pinned = 0;
+        std::vector<int> bits;
+        for (int j = 0; j < nw; j++) {
+            if (wires[j] < 0 || wires[j] >= n()) {
+                throw InvalidArg("wire index out of range");
+            }
+            m.bitpos[j] = n() - 1 - wires[j];
+            if (pinned & (uint64_t{1} << m.bitpos[j])) {
+                throw InvalidArg("wires must be distinct");
+            }
+            pinned |= uint64_t{1} << m.bitpos[j];
+            bits.push_back(m.bitpos[j]);
+        }
+        std::sort(bits.begin(), bits.end());
+        m.free_mask = (len() - 1) & ~pinned;
+        const uint64_t nbins = uint64_t{1} << nw, nrest = len() >> nw;
+        ScratchScope sc(h);
+        auto *d = static_cast<double *>(sc.get());
+        if (nw <= 12 && nrest >= 4096) {
+            m.rest.nb = nw;
+            m.rest.fixed = 0;
+            for (int j = 0; j < nw; j++) {
+                m.rest.pos[j] = bits[j];
+            }
+            const unsigned chunks = static_cast<unsigned>(std::max<uint64_t>(
+                1, std::min<uint64_t>((nrest + kBlock * 64 - 1) / (kBlock * 64),
+                                      std::max<uint64_t>(1, uint64_t(device_cus()) * 16 / nbins))));
+            double *part = reduce_buf(h, nbins * chunks * sizeof(double));
+            hipLaunchKernelGGL((k_marginal<T>), dim3(chunks, static_cast<unsigned>(nbins)),
+                               dim3(kBlock), 0, h->stream, psi(), nrest, m, part);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_marginal_final, dim3((nbins + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               h->stream, part, nbins, (int)chunks, d);
+        } else {
+            hipLaunchKernelGGL((k_marginal_small<T>), dim3((nbins + kBlock - 1) / kBlock),
+                               dim3(kBlock), 0, h->stream, psi(), nbins, nrest, m, d);
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(out, d, nbins * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+
+    // Inverse-CDF sampling (kernels_sample.hpp).  Either host uniforms
+    // (-> indices) or a SplitMix64 seed (-> bit rows).
+    void sample(const double *host_u, uint64_t seed, int64_t m, int64_t *host_idx,
+                uint64_t *host_bits) {
+        const int sb = static_cast<int>(std::min<uint64_t>(kSampleBlock, len()));
+        const uint64_t nsb = (len() + sb - 1) / sb;
+        double *du = nullptr, *bsum = nullptr, *prefix = nullptr;
+        int64_t *didx = nullptr;
+        uint64_t *dbits = nullptr;
+        struct Free {
+            std::vector<void *> p;
+            ~Free() {
+                for (void *q : p) {
+                    (void)hipFree(q);
+                }
+            }
+        } fr;
+        HIP_OK(hipMalloc(&du, m * sizeof(double)));
+        fr.p.push_back(du);
+        HIP_OK(hipMalloc(&didx, m * sizeof(int64_t)));
+        fr.p.push_back(didx);
+        HIP_OK(hipMalloc(&bsum, nsb * sizeof(double)));
+        fr.p.push_back(bsum);
+        HIP_OK(hipMalloc(&prefix, (nsb + 1) * sizeof(double)));
+        fr.p.push_back(prefix);
+        if (host_u) {
+            HIP_OK(hipMemcpyAsync(du, host_u, m * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        } else {
+            hipLaunchKernelGGL(k_uniforms, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               h->stream, seed, m, du);
+            HIP_OK(hipGetLastError());
+        }
+        const unsigned waves_per_block = kBlock / 64;
+        hipLaunchKernelGGL((k_sample_blocksum<T>), dim3((nsb + waves_per_block - 1) / waves_per_block),
+                           dim3(kBlock), 0, h->stream, psi(), len(), sb, nsb, bsum);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, h->stream, bsum, nsb, prefix);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL((k_sample_search<T>), dim3((m + waves_per_block - 1) / waves_per_block),
+                           dim3(kBlock), 0, h->stream, psi(), len(), sb, nsb, prefix, du, m, didx);
+        HIP_OK(hipGetLastError());
+        if (host_idx) {
+            HIP_OK(hipMemcpyAsync(host_idx, didx, m * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                  h->stream));
+        }
+        if (host_bits) {
+            HIP_OK(hipMalloc(&dbits, m * n() * sizeof(uint64_t)));
+            fr.p.push_back(dbits);
+            const uint64_t tot = uint64_t(m) * n();
+            hipLaunchKernelGGL(k_sample_bits, dim3((tot + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                               h->stream, didx, m, n(), dbits);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(host_bits, dbits, tot * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  h->stream));
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+
+    // Measurements::expval(Observable) and var(Observable)
+    double expval(const ObsNode &o, bool var) {
+        PauliWord pw;
+        if (!var) {
+            if (as_pauli(o, n(), pw)) {
+                return pauli_expval(psi(), {pw}, {1.0});
+            }
+            if (o.kind == 3) {
+                std::vector<PauliWord> ws(o.kids.size());
+                bool all = !o.kids.empty();
+                for (size_t t = 0; t < o.kids.size() && all; t++) {
+                    all = as_pauli(o.kids[t], n(), ws[t]);
+                }
+                if (all) {
+                    return pauli_expval(psi(), ws, o.coeffs);
+                }
+            }
+        }
+        ScratchScope sc(h);
+        V *bra = static_cast<V *>(sc.get());
+        obs_into(bra, psi(), o);
+        if (!var) {
+            return dot(bra, psi()).real();
+        }
+        const double ms = dot(bra, bra).real();
+        const double e = dot(bra, psi()).real();
+        return ms - e * e;
+    }
+};
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return PL_OK;
+    } catch (const OutOfRange &e) {
+        g_err = e.what();
+        return PL_ERR_RANGE;
+    } catch (const InvalidArg &e) {
+        g_err = e.what();
+        return PL_ERR_INVALID;
+    } catch (const DeviceError &e) {
+        g_err = e.what();
+        return PL_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return PL_ERR_INVALID;
+    }
+}
+
+template <class F> int dispatch(pl_sv_t h, F &&f) {
+    return guarded([&] {
+        if (!h) {
+            throw InvalidArg("null state-vector handle");
+        }
+        HIP_OK(hipSetDevice(0));
+        if (h->dtype == PL_C128) {
+            Engine<double> e(h);
+            f(e);
+        } else {
+            Engine<float> e(h);
+            f(e);
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// adjoint sweep (AdjointJacobianLQubit.hpp:212-336)
+// ---------------------------------------------------------------------------
+template <class T>
+void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const int *wires,
+                   const uint8_t *inverse, const int *nparams, const double *params,
+                   const int *prog, const double *coeffs, const double *mats, int ntp,
+                   const int *tp, double *jac_out) {
+    using V = v2<T>;
+    pl_sv *h = E.h;
+    struct Op {
+        int id;
+        std::vector<int> w;
+        bool inv;
+        std::vector<double> p;
+    };
+    std::vector<Op> ops(nops);
+    for (int i = 0, wo = 0, po = 0; i < nops; i++) {
+        ops[i].id = ids[i];
+        ops[i].w.assign(wires + wo, wires + wo + nwires[i]);
+        wo += nwires[i];
+        ops[i].inv = inverse[i] != 0;
+        ops[i].p.assign(params + po, params + po + nparams[i]);
+        po += nparams[i];
+        if (ops[i].id != -1) {
+            gate_name(ops[i].id);
+        }
+    }
+    ObsReader rd{prog, coeffs, mats};
+    const int K = *rd.prog++;
+    std::vector<ObsNode> obs;
+    for (int k = 0; k < K; k++) {
+        obs.push_back(rd.one());
+    }
+    if (ntp == 0) {
+        return; // JacobianData::hasTrainableParams() false: nothing to do
+    }
+    for (int t = 1; t < ntp; t++) {
+        if (tp[t] <= tp[t - 1]) {
+            throw InvalidArg("trainable parameter indices must be sorted ascending");
+        }
+    }
+    if (K == 0) {
+        return;
+    }
+    ScratchScope sc(h);
+    V *lam = static_cast<V *>(sc.get());
+    E.copy(lam, E.psi());
+    std::vector<V *> states{lam};
+    for (int k = 0; k < K; k++) {
+        V *hk = static_cast<V *>(sc.get());
+        E.obs_into(hk, lam, obs[k]);
+        states.push_back(hk);
+    }
+    // device tables: state pointers, jacobian, partials
+    V **dstates = nullptr;
+    double *djac = nullptr, *dpart = nullptr;
+    const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 8), kReduceBlocks);
+    HIP_OK(hipMalloc(&dstates, states.size() * sizeof(V *)));
+    HIP_OK(hipMalloc(&djac, size_t(ntp) * K * sizeof(double)));
+    HIP_OK(hipMalloc(&dpart, size_t(grid) * kAdjMaxObs * sizeof(double)));
+    struct Free {
+        void *a, *b, *c;
+        ~Free() {
+            (void)hipFree(a);
+            (void)hipFree(b);
+            (void)hipFree(c);
+        }
+    } fr{dstates, djac, dpart};
+    HIP_OK(hipMemcpyAsync(dstates, states.data(), states.size() * sizeof(V *),
+                          hipMemcpyHostToDevice, E.h->stream));
+    HIP_OK(hipMemsetAsync(djac, 0, size_t(ntp) * K * sizeof(double), E.h->stream));
+
+    size_t num_par_ops = 0;
+    for (auto &o : ops) {
+        num_par_ops += o.p.empty() ? 0 : 1;
+    }
+    int t_num = ntp - 1;
+    long long cur_param = static_cast<long long>(num_par_ops) - 1;
+    int tp_pos = ntp - 1;
+    ScratchScope sc2(h);
+    V *mu = nullptr;
+    for (int i = nops - 1; i >= 0; i--) {
+        const Op &op = ops[i];
+        if (op.p.size() > 1) {
+            throw InvalidArg("The operation is not supported using the adjoint differentiation method");
+        }
+        if (op.id == -1) {
+            continue;
+        }
+        if (tp_pos < 0) {
+            break;
+        }
+        const auto pt = E.params_t(op.p.data(), static_cast<int>(op.p.size()));
+        const GroupOp<T> ud = lower_gate<T>(E.n(), op.id, op.w.data(), static_cast<int>(op.w.size()),
+                                            !op.inv, pt.data(), static_cast<int>(pt.size()));
+        bool train = false;
+        GroupOp<T> g;
+        double factor = 0;
+        int row = 0;
+        if (!op.p.empty()) {
+            if (cur_param == tp[tp_pos]) {
+                train = true;
+                T s = 0;
+                g = lower_generator<T>(E.n(), op.id, op.w.data(), static_cast<int>(op.w.size()),
+                                       true, &s);
+                factor = -2.0 * static_cast<double>(s) * (op.inv ? -1.0 : 1.0);
+                row = t_num * K;
+                t_num--;
+                tp_pos--;
+            }
+            cur_param--;
+        }
+        const bool fusable =
+            ud.kind == GroupOp<T>::Noop ||
+            (ud.kind == GroupOp<T>::Parity && (!train || g.same_geometry(ud))) ||
+            (ud.kind == GroupOp<T>::Group && ud.M <= 4 && (!train || g.same_geometry(ud)));
+        if (!fusable) {
+            // unfused: mu = lambda; lambda <- U^dag lambda; mu <- G mu; dots; H_k <- U^dag H_k
+            if (train) {
+                if (!mu) {
+                    mu = static_cast<V *>(sc2.get());
+                }
+                E.copy(mu, lam);
+            }
+            E.apply(ud, lam);
+            if (train) {
+                T s = 0;
+                E.apply(lower_generator<T>(E.n(), op.id, op.w.data(), static_cast<int>(op.w.size()),
+                                           false, &s),
+                        mu);
+                std::vector<double> row_vals(K);
+                for (int k = 0; k < K; k++) {
+                    row_vals[k] = factor * E.dot(states[1 + k], mu).imag();
+                }
+                HIP_OK(hipMemcpyAsync(djac + row, row_vals.data(), K * sizeof(double),
+                                      hipMemcpyHostToDevice, E.h->stream));
+                HIP_OK(hipStreamSynchronize(E.h->stream));
+            }
+            for (int k = 0; k < K; k++) {
+                E.apply(ud, states[1 + k]);
+            }
+            continue;
+        }
+        if (ud.kind == GroupOp<T>::Noop) {
+            // Identity: U^dag = I, no generator
+            continue;
+        }
+        // fused launches, kAdjMaxObs observables each; lambda updated by the last
+        for (int k0 = 0; k0 < K; k0 += kAdjMaxObs) {
+            const int nk = std::min(kAdjMaxObs, K - k0);
+            const int upd = (k0 + nk == K) ? 1 : 0;
+            // states for this launch: lambda followed by H_k0 .. H_k0+nk-1
+            V *const *st = dstates;
+            std::vector<V *> local;
+            if (k0 > 0) {
+                // device pointer table is contiguous: shift by k0 and put lambda first
+                local.push_back(lam);
+                for (int k = 0; k < nk; k++) {
+                    local.push_back(states[1 + k0 + k]);
+                }
+                st = static_cast<V *const *>(stage(h, local.data(), local.size() * sizeof(V *)));
+            }
+            if (ud.kind == GroupOp<T>::Parity) {
+                const dcx<T> u0{ud.parity_phase[0].real(), ud.parity_phase[0].imag()};
+                const dcx<T> u1{ud.parity_phase[1].real(), ud.parity_phase[1].imag()};
+                dcx<T> g0{1, 0}, g1{1, 0};
+                if (train) {
+                    g0 = {g.parity_phase[0].real(), g.parity_phase[0].imag()};
+                    g1 = {g.parity_phase[1].real(), g.parity_phase[1].imag()};
+                    hipLaunchKernelGGL((k_adjoint_parity<T, true>), dim3(grid), dim3(kBlock), 0,
+                                       E.h->stream, st, nk, E.len(), ud.parity_mask, u0, u1, g0,
+                                       g1, upd, dpart);
+                } else {
+                    hipLaunchKernelGGL((k_adjoint_parity<T, false>), dim3(grid), dim3(kBlock), 0,
+                                       E.h->stream, st, nk, E.len(), ud.parity_mask, u0, u1, g0,
+                                       g1, upd, dpart);
+                }
+            } else {
+                auto go = [&](auto mtag) {
+                    constexpr int M = decltype(mtag)::value;
+                    AdjOp<T, M> a{};
+                    a.geo.fixed = ud.fixed;
+                    a.geo.nb = ud.nb;
+                    for (int q = 0; q < ud.nb; q++) {
+                        a.geo.pos[q] = ud.pos[q];
+                    }
+                    for (int j = 0; j < M; j++) {
+                        a.off[j] = ud.off[j];
+                    }
+                    for (int j = 0; j < M * M; j++) {
+                        a.ud[j] = {ud.mat[j].real(), ud.mat[j].imag()};
+                        a.g[j] = train ? dcx<T>{g.mat[j].real(), g.mat[j].imag()} : dcx<T>{0, 0};
+                    }
+                    a.ud_diag = ud.diag ? 1 : 0;
+                    const uint64_t nwork = E.len() >> ud.nb;
+                    const unsigned gr = std::min<unsigned>(grid_for(nwork, 1, 8), kReduceBlocks);
+                    if (train) {
+                        hipLaunchKernelGGL((k_adjoint_step<T, M, true>), dim3(gr), dim3(kBlock), 0,
+                                           E.h->stream, st, nk, nwork, a, upd, dpart);
+                    } else {
+                        hipLaunchKernelGGL((k_adjoint_step<T, M, false>), dim3(gr), dim3(kBlock), 0,
+                                           E.h->stream, st, nk, nwork, a, upd, dpart);
+                    }
+                    return gr;
+                };
+                unsigned gr = 0;
+                switch (ud.M) {
+                case 1: gr = go(std::integral_constant<int, 1>{}); break;
+                case 2: gr = go(std::integral_constant<int, 2>{}); break;
+                default: gr = go(std::integral_constant<int, 4>{}); break;
+                }
+                if (train) {
+                    HIP_OK(hipGetLastError());
+                    hipLaunchKernelGGL(k_adjoint_final, dim3(1), dim3(64), 0, E.h->stream, dpart,
+                                       (int)gr, nk, factor, djac + row + k0);
+                }
+                HIP_OK(hipGetLastError());
+                continue;
+            }
+            HIP_OK(hipGetLastError());
+            if (train) {
+                hipLaunchKernelGGL(k_adjoint_final, dim3(1), dim3(64), 0, E.h->stream, dpart,
+                                   (int)grid, nk, factor, djac + row + k0);
+                HIP_OK(hipGetLastError());
+            }
+        }
+    }
+    std::vector<double> J(size_t(ntp) * K);
+    HIP_OK(hipMemcpyAsync(J.data(), djac, J.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          E.h->stream));
+    HIP_OK(hipStreamSynchronize(E.h->stream));
+    // parameter-major -> observable-major (AdjointJacobianLQubit.hpp:331-334)
+    for (int k = 0; k < K; k++) {
+        for (int t = 0; t < ntp; t++) {
+            jac_out[size_t(k) * ntp + t] = J[size_t(t) * K + k];
+        }
+    }
+}
+
+} // namespace plamd
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C" {
+
+const char *pl_last_error(void) { return g_err.c_str(); }
+
+int pl_device_info(char *buf, size_t len) {
+    return guarded([&] {
+        int dev = 0;
+        hipDeviceProp_t p;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipGetDeviceProperties(&p, dev));
+        std::snprintf(buf, len, "{\"name\": \"%s\", \"arch\": \"%s\", \"cus\": %d, \"hbm_bytes\": %zu}",
+                      p.name, p.gcnArchName, p.multiProcessorCount, p.totalGlobalMem);
+    });
+}
+
+int pl_sv_create(int num_qubits, int dtype, pl_sv_t *out) {
+    return guarded([&] {
+        if (!out) {
+            throw InvalidArg("null output handle");
+        }
+        if (num_qubits < 1 || num_qubits > 40) {
+            throw InvalidArg("num_qubits must be in [1, 40]");
+        }
+        if (dtype != PL_C64 && dtype != PL_C128) {
+            throw InvalidArg("dtype must be PL_C64 or PL_C128");
+        }
+        HIP_OK(hipSetDevice(0));
+        auto h = std::make_unique<pl_sv>();
+        h->n = num_qubits;
+        h->dtype = dtype;
+        h->len = uint64_t{1} << num_qubits;
+        HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&h->ev0));
+        HIP_OK(hipEventCreate(&h->ev1));
+        hipError_t e = hipMalloc(&h->d, h->state_bytes());
+        if (e != hipSuccess) {
+            (void)hipStreamDestroy(h->stream);
+            throw DeviceError(std::string("hipMalloc of the state failed: ") + hipGetErrorString(e));
+        }
+        pl_sv *raw = h.release();
+        if (pl_sv_reset(raw) != PL_OK) {
+            const std::string msg = g_err;
+            pl_sv_destroy(raw);
+            throw DeviceError(msg);
+        }
+        *out = raw;
+    });
+}
+
+int pl_sv_destroy(pl_sv_t h) {
+    return guarded([&] {
+        if (!h) {
+            return;
+        }
+        (void)hipStreamSynchronize(h->stream);
+        for (void *p : h->pool) {
+            (void)hipFree(p);
+        }
+        (void)hipFree(h->red);
+        (void)hipFree(h->staging);
+        (void)hipFree(h->d);
+        for (hipEvent_t e : h->slots) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        (void)hipEventDestroy(h->ev0);
+        (void)hipEventDestroy(h->ev1);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+    });
+}
+
+int pl_sv_num_qubits(pl_sv_t h, int *out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        *out = h->n;
+    });
+}
+int pl_sv_dtype(pl_sv_t h, int *out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        *out = h->dtype;
+    });
+}
+
+int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {
+    return guarded([&] {
+        if (!h || !host) throw InvalidArg("null argument");
+        if (nbytes != h->state_bytes()) {
+            throw InvalidArg("upload size does not match the state vector size");
+        }
+        HIP_OK(hipMemcpyAsync(h->d, host, nbytes, hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    });
+}
+int pl_sv_download(pl_sv_t h, void *host, size_t nbytes) {
+    return guarded([&] {
+        if (!h || !host) throw InvalidArg("null argument");
+        if (nbytes != h->state_bytes()) {
+            throw InvalidArg("download size does not match the state vector size");
+        }
+        HIP_OK(hipMemcpyAsync(host, h->d, nbytes, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    });
+}
+int pl_sv_copy(pl_sv_t dst, pl_sv_t src) {
+    return guarded([&] {
+        if (!dst || !src) throw InvalidArg("null argument");
+        if (dst->n != src->n || dst->dtype != src->dtype) {
+            throw InvalidArg("state vectors differ in size or precision");
+        }
+        HIP_OK(hipStreamSynchronize(src->stream));
+        HIP_OK(hipMemcpyAsync(dst->d, src->d, dst->state_bytes(), hipMemcpyDeviceToDevice, dst->stream));
+    });
+}
+int pl_sv_set_basis_state(pl_sv_t h, uint64_t index) {
+    return guarded([&] {
+        if (!h) throw InvalidArg("null argument");
+        if (index >= h->len) throw InvalidArg("basis index out of range");
+        const unsigned grid = grid_for(h->len);
+        if (h->dtype == PL_C128) {
+            hipLaunchKernelGGL((k_fill_basis<double>), dim3(grid), dim3(kBlock), 0, h->stream,
+                               static_cast<double2 *>(h->d), h->len, index);
+        } else {
+            hipLaunchKernelGGL((k_fill_basis<float>), dim3(grid), dim3(kBlock), 0, h->stream,
+                               static_cast<float2 *>(h->d), h->len, index);
+        }
+        HIP_OK(hipGetLastError());
+    });
+}
+int pl_sv_reset(pl_sv_t h) { return pl_sv_set_basis_state(h, 0); }
+int pl_sv_device_ptr(pl_sv_t h, void **out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        *out = h->d;
+    });
+}
+int pl_sv_synchronize(pl_sv_t h) {
+    return guarded([&] {
+        if (!h) throw InvalidArg("null argument");
+        HIP_OK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int pl_sv_apply_gate(pl_sv_t h, int id, const int *wires, int nw, int inverse,
+                     const double *params, int np) {
+    return dispatch(h, [&](auto &E) { E.gate(E.psi(), id, wires, nw, inverse != 0, params, np); });
+}
+
+int pl_sv_apply_ops(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
+                    const uint8_t *inverse, const int *nparams, const double *params) {
+    return dispatch(h, [&](auto &E) {
+        for (int i = 0, wo = 0, po = 0; i < nops; i++) {
+            if (ids[i] != -1) {
+                E.gate(E.psi(), ids[i], wires + wo, nwires[i], inverse[i] != 0, params + po,
+                       nparams[i]);
+            }
+            wo += nwires[i];
+            po += nparams[i];
+        }
+    });
+}
+
+int pl_sv_apply_matrix(pl_sv_t h, const double *m, const int *wires, int nw, int inverse) {
+    return dispatch(h, [&](auto &E) { E.matrix(E.psi(), m, wires, nw, inverse != 0); });
+}
+
+int pl_sv_apply_generator(pl_sv_t h, int id, const int *wires, int nw, int adj, double *scale) {
+    (void)adj; // the generators ignore `adj` (GateImplementationsLM.hpp:1355-1640)
+    return dispatch(h, [&](auto &E) {
+        const double s = E.generator(E.psi(), id, wires, nw);
+        if (scale) {
+            *scale = s;
+        }
+    });
+}
+
+int pl_expval(pl_sv_t h, const int *prog, const double *coeffs, const double *mats, double *out) {
+    return dispatch(h, [&](auto &E) {
+        ObsReader rd{prog + 1, coeffs, mats};
+        *out = E.expval(rd.one(), false);
+    });
+}
+int pl_var(pl_sv_t h, const int *prog, const double *coeffs, const double *mats, double *out) {
+    return dispatch(h, [&](auto &E) {
+        ObsReader rd{prog + 1, coeffs, mats};
+        *out = E.expval(rd.one(), true);
+    });
+}
+
+int pl_expval_named(pl_sv_t h, int id, const int *wires, int nw, double *out) {
+    return dispatch(h, [&](auto &E) {
+        ObsNode o;
+        o.kind = 0;
+        o.op = id;
+        gate_name(id);
+        o.wires.assign(wires, wires + nw);
+        *out = E.expval(o, false);
+    });
+}
+int pl_var_named(pl_sv_t h, int id, const int *wires, int nw, double *out) {
+    return dispatch(h, [&](auto &E) {
+        ObsNode o;
+        o.kind = 0;
+        o.op = id;
+        gate_name(id);
+        o.wires.assign(wires, wires + nw);
+        *out = E.expval(o, true);
+    });
+}
+int pl_expval_matrix(pl_sv_t h, const double *m, const int *wires, int nw, double *out) {
+    return dispatch(h, [&](auto &E) {
+        if (nw <= 0 || nw > kMaxPos) {
+            throw InvalidArg("matrix observable needs 1..12 wires");
+        }
+        ObsNode o;
+        o.kind = 1;
+        o.wires.assign(wires, wires + nw);
+        const size_t dim = size_t{1} << (2 * nw);
+        for (size_t i = 0; i < dim; i++) {
+            o.mat.emplace_back(m[2 * i], m[2 * i + 1]);
+        }
+        *out = E.expval(o, false);
+    });
+}
+
+int pl_probs(pl_sv_t h, const int *wires, int nw, double *out) {
+    return dispatch(h, [&](auto &E) {
+        if (!out) throw InvalidArg("null output");
+        E.probs(wires, nw, out);
+    });
+}
+
+int pl_sample(pl_sv_t h, const double *uniforms, int64_t m, int64_t *idx) {
+    return dispatch(h, [&](auto &E) {
+        if (m < 0) throw InvalidArg("negative sample count");
+        if (m == 0) return;
+        if (!uniforms || !idx) throw InvalidArg("null argument");
+        E.sample(uniforms, 0, m, idx, nullptr);
+    });
+}
+
+int pl_generate_samples(pl_sv_t h, uint64_t seed, int64_t m, uint64_t *bits) {
+    return dispatch(h, [&](auto &E) {
+        if (m < 0) throw InvalidArg("negative sample count");
+        if (m == 0) return;
+        if (!bits) throw InvalidArg("null argument");
+        E.sample(nullptr, seed, m, nullptr, bits);
+    });
+}
+
+int pl_adjoint_jacobian(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
+                        const uint8_t *inverse, const int *nparams, const double *params,
+                        const int *prog, const double *coeffs, const double *mats, int ntp,
+                        const int *tp, double *jac) {
+    return dispatch(h, [&](auto &E) {
+        adjoint_sweep(E, nops, ids, nwires, wires, inverse, nparams, params, prog, coeffs, mats,
+                      ntp, tp, jac);
+    });
+}
+
+int pl_timer_start(pl_sv_t h) {
+    return guarded([&] {
+        if (!h) throw InvalidArg("null argument");
+        HIP_OK(hipEventRecord(h->ev0, h->stream));
+    });
+}
+int pl_timer_stop(pl_sv_t h, double *ms) {
+    return guarded([&] {
+        if (!h || !ms) throw InvalidArg("null argument");
+        HIP_OK(hipEventRecord(h->ev1, h->stream));
+        HIP_OK(hipEventSynchronize(h->ev1));
+        float f = 0;
+        HIP_OK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+        *ms = f;
+    });
+}
+
+int pl_event_record(pl_sv_t h, int slot) {
+    return guarded([&] {
+        if (!h) throw InvalidArg("null argument");
+        if (slot < 0 || slot >= 65536) throw InvalidArg("event slot out of range");
+        if (h->slots.size() <= static_cast<size_t>(slot)) {
+            h->slots.resize(slot + 1, nullptr);
+        }
+        if (!h->slots[slot]) {
+            HIP_OK(hipEventCreate(&h->slots[slot]));
+        }
+        HIP_OK(hipEventRecord(h->slots[slot], h->stream));
+    });
+}
+int pl_event_elapsed(pl_sv_t h, int a, int b, double *ms) {
+    return guarded([&] {
+        if (!h || !ms) throw InvalidArg("null argument");
+        if (a < 0 || b < 0 || static_cast<size_t>(std::max(a, b)) >= h->slots.size() ||
+            !h->slots[a] || !h->slots[b]) {
+            throw InvalidArg("event slot was never recorded");
+        }
+        HIP_OK(hipEventSynchronize(h->slots[b]));
+        float f = 0;
+        HIP_OK(hipEventElapsedTime(&f, h->slots[a], h->slots[b]));
+        *ms = f;
+    });
+}
+
+} // extern "C"
