@@ -30,15 +30,19 @@ from lightning_unification_amd.gates import touched_fraction  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
-# kernel instantiation each gate of the workload lowers to (lower.hpp)
-KERNEL_OF = {
-    "Hadamard": "k_group_small<double,2,2>", "RX": "k_group_small<double,2,2>",
-    "RY": "k_group_small<double,2,2>", "RZ": "k_group_small<double,2,2>",
-    "Rot": "k_group_small<double,2,2>", "CRX": "k_group_small<double,2,2>",
-    "CRZ": "k_group_small<double,2,2>", "SWAP": "k_group_small<double,2,2>",
-    "Toffoli": "k_group_small<double,2,2>", "T": "k_group_small<double,1,2>",
-    "IsingXX": "k_group_small<double,4,1>", "IsingZZ": "k_group_small<double,4,1>",
-}
+def kernel_for(name, wires, n):
+    """Kernel instantiation a gate of this workload runs as (the dispatch of
+    Engine::launch_small in csrc/pl_amd.hip)."""
+    bits = [n - 1 - w for w in wires]
+    m = 1 if name == "T" else (4 if name in ("IsingXX", "IsingZZ") else 2)
+    if m == 2 and name != "SWAP":
+        target, others = bits[-1], bits[:-1]  # 1q gates, CRX/CRZ, Toffoli
+        if target < 6 and all(b >= 6 for b in others) and n - len(others) >= 6:
+            return "k_pair_lane<double>"
+    nhi = sum(1 for b in bits if b >= 8)
+    if min(bits) < 3 and nhi <= 4 and n >= 8 + nhi:
+        return f"k_group_tile<double,{m}>"
+    return f"k_group_flat<double,{m}>"
 
 
 def log(msg):
@@ -85,6 +89,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--shots", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["gates", "copy"], default="gates",
+                    help="copy: device-to-device copies of a 4 GiB state, the plain copy "
+                         "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
@@ -92,6 +99,14 @@ def main():
         sys.exit(2)
 
     from lightning_unification_amd import capi
+    if args.mode == "copy":
+        a = capi.DeviceStateVector(28, np.complex128)
+        b = capi.DeviceStateVector(28, np.complex128)
+        for _ in range(3):
+            b.copy_from(a)
+        b.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_per_copy": (1 << 28) * 16, "copies": 3}))
+        return
     n = args.qubits
     tape = random_layers(n, 50, seed=1002)
     ham = random_hamiltonian(n, 100, seed=1002)
@@ -132,7 +147,7 @@ def main():
     # dominant kernel: sum time and algorithmic bytes per kernel instantiation
     by_kernel = {}
     for i, op in enumerate(tape.ops):
-        k = KERNEL_OF.get(op[0], op[0])
+        k = kernel_for(op[0], op[1], n)
         d = by_kernel.setdefault(k, {"ms": 0.0, "bytes": 0.0, "launches": 0, "gates": set()})
         d["ms"] += per_gate_ms[i]
         d["bytes"] += gate_bytes[i] * args.steps

@@ -105,6 +105,12 @@ int pl_sv_apply_matrix(pl_sv_t sv, const double *matrix, const int *wires,
 int pl_sv_apply_generator(pl_sv_t sv, int gate_id, const int *wires,
                           int nwires, int adj, double *scale);
 
+/* Observable::applyInPlace (core/src/observables/Observables.hpp:98-324,
+ * ObservablesLQubit.hpp:351-355 for Hamiltonians): sv <- O sv for the one
+ * observable held by prog/coeffs/mats. */
+int pl_sv_apply_observable(pl_sv_t sv, const int *prog, const double *coeffs,
+                           const double *mats);
+
 /* ---- measurements ---------------------------------------------------------- */
 
 /* Measurements::expval(Observable) / var(Observable)
@@ -122,7 +128,10 @@ int pl_var_named(pl_sv_t sv, int gate_id, const int *wires, int nwires,
 int pl_expval_matrix(pl_sv_t sv, const double *matrix, const int *wires,
                      int nwires, double *out);
 /* Measurements::probs() (nwires < 0, out has 2^n entries) and probs(wires)
- * (out has 2^nwires entries, caller's wire order), MeasurementsLQubit.hpp:79-135. */
+ * (out has 2^nwires entries), MeasurementsLQubit.hpp:79-135.  As in the
+ * reference, output axis j (most significant first) is wire
+ * sorted(wires)[argsort(wires)[j]]: the caller's order whenever that order's
+ * permutation is its own inverse (sorted, reversed, one swap). */
 int pl_probs(pl_sv_t sv, const int *wires, int nwires, double *out);
 /* Inverse-CDF sampling with caller-supplied uniforms in [0,1): idx[s] is the
  * basis index with C_i < u_s <= C_{i+1} (C exclusive prefix of |psi|^2 in

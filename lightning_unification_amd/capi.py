@@ -26,7 +26,7 @@ EXPORTS = [
     "pl_sv_apply_ops", "pl_sv_apply_matrix", "pl_sv_apply_generator", "pl_expval", "pl_var",
     "pl_expval_named", "pl_var_named", "pl_expval_matrix", "pl_probs", "pl_sample",
     "pl_generate_samples", "pl_adjoint_jacobian", "pl_timer_start", "pl_timer_stop",
-    "pl_event_record", "pl_event_elapsed",
+    "pl_event_record", "pl_event_elapsed", "pl_sv_apply_observable",
 ]
 
 _lib = None
@@ -174,6 +174,10 @@ class DeviceStateVector:
     def _obs(self, ob):
         prog, coeffs, mats = encode_observables([ob])
         return prog, coeffs, np.ascontiguousarray(mats).view(np.float64)
+
+    def apply_observable(self, ob):
+        prog, coeffs, mats = self._obs(ob)
+        _check(load().pl_sv_apply_observable(self.h, _p(prog), _p(coeffs), _p(mats)))
 
     def expval(self, ob):
         prog, coeffs, mats = self._obs(ob)

@@ -66,49 +66,171 @@ template <class T, int M> struct SmallOp {
 
 constexpr int kBlock = 256;
 
-// Small group op, UNROLL independent groups per thread per iteration (their
-// loads are issued back to back to keep enough bytes in flight).
-template <class T, int M, int UNROLL>
-__global__ __launch_bounds__(kBlock) void k_group_small(v2<T> *__restrict__ sv, uint64_t nwork,
-                                                        const SmallOp<T, M> op) {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock * UNROLL;
-    for (uint64_t g0 = static_cast<uint64_t>(blockIdx.x) * kBlock * UNROLL + threadIdx.x; g0 < nwork;
-         g0 += stride) {
-        uint64_t base[UNROLL];
-        v2<T> v[UNROLL][M];
+// Streaming loads/stores: each amplitude is touched once per gate, so the
+// accesses are marked non-temporal (no L2/MALL retention).
+template <class T> __device__ __forceinline__ v2<T> ld_nt(const v2<T> *p) {
+    v2<T> v;
+    v.x = __builtin_nontemporal_load(&p->x);
+    v.y = __builtin_nontemporal_load(&p->y);
+    return v;
+}
+template <class T> __device__ __forceinline__ void st_nt(v2<T> *p, v2<T> v) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that XCD x
+// walks one contiguous eighth of the index space (when the grid divides).
+__device__ __forceinline__ uint64_t xcd_block() {
+    const uint64_t b = blockIdx.x, nb = gridDim.x;
+    if (nb & 7) {
+        return b;
+    }
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// One-shot group op: thread per group, grid covers the work exactly once.
+template <class T, int M>
+__global__ __launch_bounds__(kBlock) void k_group_flat(v2<T> *__restrict__ sv, uint64_t nwork,
+                                                       const SmallOp<T, M> op) {
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g >= nwork) {
+        return;
+    }
+    const uint64_t base = expand(g, op.geo);
+    v2<T> v[M];
 #pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            const uint64_t g = g0 + static_cast<uint64_t>(u) * kBlock;
-            base[u] = g < nwork ? expand(g, op.geo) : ~uint64_t{0};
-            if (base[u] != ~uint64_t{0}) {
+    for (int j = 0; j < M; j++) {
+        v[j] = ld_nt<T>(sv + (base | op.off[j]));
+    }
+    if (op.diag) {
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    v[u][j] = sv[base[u] | op.off[j]];
-                }
+        for (int j = 0; j < M; j++) {
+            st_nt<T>(sv + (base | op.off[j]), cmul(op.mat[j * M + j], v[j]));
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < M; r++) {
+            v2<T> acc = cmul(op.mat[r * M], v[0]);
+#pragma unroll
+            for (int c = 1; c < M; c++) {
+                cfma(acc, op.mat[r * M + c], v[c]);
+            }
+            st_nt<T>(sv + (base | op.off[r]), acc);
+        }
+    }
+}
+
+// 2x2 op on target bit p < 6 with every other pinned bit >= 6: lane l owns
+// one amplitude (fully coalesced wavefront loads), its partner lives in lane
+// l ^ (1 << p) and is fetched with a cross-lane shuffle.  `geo` pins the
+// control bits only; the target bit is a free bit of the thread index.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pair_lane(v2<T> *__restrict__ sv, uint64_t nelem,
+                                                      Geom geo, int p, dcx<T> m00, dcx<T> m01,
+                                                      dcx<T> m10, dcx<T> m11, int diag) {
+    const uint64_t t = xcd_block() * kBlock + threadIdx.x;
+    // nelem is a multiple of 64 here, so whole wavefronts are in or out
+    if (t >= nelem) {
+        return;
+    }
+    const uint64_t i = expand(t, geo);
+    const v2<T> v = ld_nt<T>(sv + i);
+    v2<T> o;
+    o.x = __shfl_xor(v.x, 1 << p, 64);
+    o.y = __shfl_xor(v.y, 1 << p, 64);
+    const bool hi = (t >> p) & 1;
+    v2<T> r;
+    if (diag) {
+        r = cmul(hi ? m11 : m00, v);
+    } else if (hi) {
+        r = cmul(m10, o);
+        cfma(r, m11, v);
+    } else {
+        r = cmul(m00, v);
+        cfma(r, m01, o);
+    }
+    st_nt<T>(sv + i, r);
+}
+
+// Tiled group op for gates with a pinned bit inside a 128-byte line (bit
+// positions 0..2 for complex128), where per-group access would issue
+// partial-line loads and stores.  A workgroup stages a tile of 256
+// consecutive amplitudes times every combination of the pinned bits >= 8
+// ("high" pins, at most kTileMaxHi of them) in LDS with full-line coalesced
+// loads, applies every group that lies in the tile, and writes the whole tile
+// back, so HBM sees whole lines in both directions.
+constexpr int kTileLow = 8;   // tile spans amplitude bits 0..7
+constexpr int kTileMaxHi = 4; // LDS: 256 * 16 * 16 B = 64 KiB for complex128
+
+template <class T, int M> struct TileOp {
+    Geom hi;                 // pins >= kTileLow (ascending), no fixed bits
+    Geom lo;                 // pins < kTileLow (ascending), no fixed bits
+    uint32_t fixed_lo;       // fixed bits below kTileLow
+    uint32_t fixed_h;        // fixed high bits, as bits of the tile's h index
+    uint32_t off_e[M];       // member offset: bits below kTileLow
+    uint32_t off_h[M];       // member offset: high pins as h-index bits
+    dcx<T> mat[M * M];
+    int diag;
+};
+
+// NHI = number of high pins (op.hi.nb); the LDS tile holds 2^NHI * 256 amplitudes.
+template <class T, int M, int NHI>
+__global__ __launch_bounds__(kBlock) void k_group_tile(v2<T> *__restrict__ sv, uint64_t ntiles,
+                                                       const TileOp<T, M> op) {
+    __shared__ v2<T> tile[(1 << NHI) << kTileLow];
+    const uint64_t t = xcd_block();
+    if (t >= ntiles) {
+        return;
+    }
+    constexpr int nh = 1 << NHI;
+    const uint64_t e = threadIdx.x; // kBlock == 1 << kTileLow
+    const uint64_t base = expand((t << kTileLow) | e, op.hi);
+    for (int h = 0; h < nh; h++) {
+        uint64_t hb = 0;
+        for (int k = 0; k < NHI; k++) {
+            if ((h >> k) & 1) {
+                hb |= uint64_t{1} << op.hi.pos[k];
             }
         }
+        tile[(h << kTileLow) | e] = ld_nt<T>(sv + (base | hb));
+    }
+    __syncthreads();
+    const int ngroups = kBlock >> op.lo.nb;
+    for (int q = threadIdx.x; q < ngroups; q += kBlock) {
+        const uint32_t be = static_cast<uint32_t>(expand(q, op.lo)) | op.fixed_lo;
+        v2<T> v[M];
 #pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            if (base[u] == ~uint64_t{0}) {
-                continue;
+        for (int j = 0; j < M; j++) {
+            v[j] = tile[((op.fixed_h | op.off_h[j]) << kTileLow) | (be | op.off_e[j])];
+        }
+        if (op.diag) {
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                tile[((op.fixed_h | op.off_h[j]) << kTileLow) | (be | op.off_e[j])] =
+                    cmul(op.mat[j * M + j], v[j]);
             }
-            if (op.diag) {
+        } else {
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    sv[base[u] | op.off[j]] = cmul(op.mat[j * M + j], v[u][j]);
+            for (int r = 0; r < M; r++) {
+                v2<T> acc = cmul(op.mat[r * M], v[0]);
+#pragma unroll
+                for (int c = 1; c < M; c++) {
+                    cfma(acc, op.mat[r * M + c], v[c]);
                 }
-            } else {
-#pragma unroll
-                for (int r = 0; r < M; r++) {
-                    v2<T> acc = cmul(op.mat[r * M], v[u][0]);
-#pragma unroll
-                    for (int c = 1; c < M; c++) {
-                        cfma(acc, op.mat[r * M + c], v[u][c]);
-                    }
-                    sv[base[u] | op.off[r]] = acc;
-                }
+                tile[((op.fixed_h | op.off_h[r]) << kTileLow) | (be | op.off_e[r])] = acc;
             }
         }
+    }
+    __syncthreads();
+    for (int h = 0; h < nh; h++) {
+        uint64_t hb = 0;
+        for (int k = 0; k < NHI; k++) {
+            if ((h >> k) & 1) {
+                hb |= uint64_t{1} << op.hi.pos[k];
+            }
+        }
+        st_nt<T>(sv + (base | hb), tile[(h << kTileLow) | e]);
     }
 }
 

@@ -218,7 +218,7 @@ __device__ __forceinline__ uint64_t bin_bits(uint64_t bin, const MarginalGeom &m
     return r;
 }
 
-// Marginal probabilities, bins in the caller's wire order.  Grid: blockIdx.y
+// Marginal probabilities, bin bit j taken from bit position bitpos[j].  Grid: blockIdx.y
 // = bin, blockIdx.x = chunk of the remaining index space; partial[bin][chunk].
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_marginal(const v2<T> *__restrict__ sv,

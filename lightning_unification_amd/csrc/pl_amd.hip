@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -274,9 +275,84 @@ template <class T> struct Engine {
         }
         op.diag = g.diag ? 1 : 0;
         const uint64_t nwork = len() >> g.nb;
-        constexpr int U = M <= 2 ? 2 : 1;
-        hipLaunchKernelGGL((k_group_small<T, M, U>), dim3(grid_for(nwork, U)), dim3(kBlock), 0,
-                           h->stream, sv, nwork, op);
+        if constexpr (M == 2) {
+            // target bit inside a wavefront's 64 consecutive amplitudes and
+            // no other pinned bit there: lane-per-amplitude shuffle kernel
+            const uint64_t t = g.off[1];
+            const bool single = g.off[0] == 0 && t && !(t & (t - 1));
+            const int p = single ? __builtin_ctzll(t) : 64;
+            bool lane_ok = p < 6 && (nwork << 1) >= 64;
+            for (int i = 0; i < g.nb && lane_ok; i++) {
+                lane_ok = g.pos[i] == p || g.pos[i] >= 6;
+            }
+            if (lane_ok) {
+                Geom geo{};
+                geo.fixed = g.fixed;
+                for (int i = 0; i < g.nb; i++) {
+                    if (g.pos[i] != p) {
+                        geo.pos[geo.nb++] = g.pos[i];
+                    }
+                }
+                const uint64_t nelem = nwork << 1;
+                hipLaunchKernelGGL((k_pair_lane<T>), dim3(static_cast<unsigned>(nelem / kBlock + (nelem % kBlock ? 1 : 0))),
+                                   dim3(kBlock), 0, h->stream, sv, nelem, geo, p, op.mat[0], op.mat[1],
+                                   op.mat[2], op.mat[3], op.diag);
+                HIP_OK(hipGetLastError());
+                return;
+            }
+        }
+        // a pinned bit inside a 128-byte line: stage whole lines through LDS
+        constexpr int line_bits = sizeof(T) == 8 ? 3 : 4;
+        int nhi = 0;
+        for (int i = 0; i < g.nb; i++) {
+            nhi += g.pos[i] >= kTileLow ? 1 : 0;
+        }
+        if (g.nb > 0 && g.pos[0] < line_bits && nhi <= kTileMaxHi && n() >= kTileLow + nhi) {
+            TileOp<T, M> to{};
+            uint32_t hmap_pos[kTileMaxHi] = {};
+            for (int i = 0; i < g.nb; i++) {
+                if (g.pos[i] >= kTileLow) {
+                    hmap_pos[to.hi.nb] = g.pos[i];
+                    to.hi.pos[to.hi.nb++] = g.pos[i];
+                } else {
+                    to.lo.pos[to.lo.nb++] = g.pos[i];
+                }
+            }
+            auto compress_h = [&](uint64_t bits) {
+                uint32_t hbits = 0;
+                for (int k = 0; k < to.hi.nb; k++) {
+                    if ((bits >> hmap_pos[k]) & 1U) {
+                        hbits |= 1U << k;
+                    }
+                }
+                return hbits;
+            };
+            constexpr uint64_t low_mask = (uint64_t{1} << kTileLow) - 1;
+            to.fixed_lo = static_cast<uint32_t>(g.fixed & low_mask);
+            to.fixed_h = compress_h(g.fixed);
+            for (int j = 0; j < M; j++) {
+                to.off_e[j] = static_cast<uint32_t>(g.off[j] & low_mask);
+                to.off_h[j] = compress_h(g.off[j]);
+                for (int c = 0; c < M; c++) {
+                    to.mat[j * M + c] = op.mat[j * M + c];
+                }
+            }
+            to.diag = op.diag;
+            const uint64_t ntiles = len() >> (kTileLow + nhi);
+            const unsigned grid = static_cast<unsigned>(ntiles);
+            switch (nhi) {
+            case 0: hipLaunchKernelGGL((k_group_tile<T, M, 0>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            case 1: hipLaunchKernelGGL((k_group_tile<T, M, 1>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            case 2: hipLaunchKernelGGL((k_group_tile<T, M, 2>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            case 3: hipLaunchKernelGGL((k_group_tile<T, M, 3>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            default: hipLaunchKernelGGL((k_group_tile<T, M, 4>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            }
+            HIP_OK(hipGetLastError());
+            return;
+        }
+        const unsigned grid = static_cast<unsigned>((nwork + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL((k_group_flat<T, M>), dim3(grid), dim3(kBlock), 0, h->stream, sv, nwork,
+                           op);
         HIP_OK(hipGetLastError());
     }
 
@@ -537,8 +613,8 @@ template <class T> struct Engine {
     }
 
     // Measurements::probs() / probs(wires) (MeasurementsLQubit.hpp:79-135): the
-    // marginal is computed directly in the caller's wire order, which equals
-    // the reference's sorted marginal followed by transpose_state_tensor.
+    // marginal is computed directly in the reference's output axis order
+    // (its sorted marginal followed by transpose_state_tensor).
     void probs(const int *wires, int nw, double *out) {
         if (nw < 0) {
             ScratchScope sc(h); // a state buffer holds >= 2^n doubles
@@ -561,18 +637,35 @@ template <class T> struct Engine {
             if (wires[j] < 0 || wires[j] >= n()) {
                 throw InvalidArg("wire index out of range");
             }
-            m.bitpos[j] = n() - 1 - wires[j];
-            if (pinned & (uint64_t{1} << m.bitpos[j])) {
+            const int b = n() - 1 - wires[j];
+            if (pinned & (uint64_t{1} << b)) {
                 throw InvalidArg("wires must be distinct");
             }
-            pinned |= uint64_t{1} << m.bitpos[j];
-            bits.push_back(m.bitpos[j]);
+            pinned |= uint64_t{1} << b;
+            bits.push_back(b);
         }
         std::sort(bits.begin(), bits.end());
+        // Output axis j of the reference's result is wire sorted[argsort(wires)[j]]:
+        // it marginalises over the sorted wires, then applies
+        // transpose_state_tensor(probs, sorting_indices(wires))
+        // (MeasurementsLQubit.hpp:99-135, Util.hpp:440-520).
+        {
+            std::vector<int> order(nw), sorted(wires, wires + nw);
+            std::iota(order.begin(), order.end(), 0);
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return wires[a] < wires[b]; });
+            std::sort(sorted.begin(), sorted.end());
+            for (int j = 0; j < nw; j++) {
+                m.bitpos[j] = n() - 1 - sorted[order[j]];
+            }
+        }
         m.free_mask = (len() - 1) & ~pinned;
         const uint64_t nbins = uint64_t{1} << nw, nrest = len() >> nw;
-        ScratchScope sc(h);
-        auto *d = static_cast<double *>(sc.get());
+        double *d = nullptr;
+        HIP_OK(hipMalloc(reinterpret_cast<void **>(&d), nbins * sizeof(double)));
+        struct FreeD {
+            double *p;
+            ~FreeD() { (void)hipFree(p); }
+        } free_d{d};
         if (nw <= 12 && nrest >= 4096) {
             m.rest.nb = nw;
             m.rest.fixed = 0;
@@ -1140,6 +1233,14 @@ int pl_sv_apply_generator(pl_sv_t h, int id, const int *wires, int nw, int adj, 
         if (scale) {
             *scale = s;
         }
+    });
+}
+
+int pl_sv_apply_observable(pl_sv_t h, const int *prog, const double *coeffs, const double *mats) {
+    return dispatch(h, [&](auto &E) {
+        if (!prog) throw InvalidArg("null observable program");
+        ObsReader rd{prog + 1, coeffs, mats};
+        E.apply_obs(E.psi(), rd.one());
     });
 }
 

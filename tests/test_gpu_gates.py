@@ -29,7 +29,7 @@ def wire_tuples(n, k, limit=None):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [1, 3, 7])
+@pytest.mark.parametrize("n", [1, 3, 7, 11])
 def test_all_gates_all_wires(oracle, dtype, n):
     capi = _capi()
     rs = np.random.default_rng(n)

@@ -70,8 +70,9 @@ def test_probs(oracle, dtype, n):
     capi = _capi()
     s = random_state(n, 77 + n, dtype)
     sv = capi.DeviceStateVector.from_array(s)
-    for wires in [None, [0], [n - 1], [2, 0], [0, 1, 2], [n - 1, 1, 0], list(range(n)),
-                  list(reversed(range(n)))]:
+    for wires in [None, [0], [n - 1], [2, 0], [0, 1, 2], [n - 1, 1, 0], [1, 2, 0], [2, 0, 1],
+                  [3, 0, 2, 1], list(range(n)), list(reversed(range(n))),
+                  list(range(1, n)) + [0]]:
         exp = oracle.probs(s, wires)
         got = sv.probs(wires)
         assert got.shape == exp.shape
