@@ -158,6 +158,30 @@ int pl_adjoint_jacobian(pl_sv_t sv, int nops, const int *ids,
                         const double *coeffs, const double *mats, int ntp,
                         const int *tp, double *jac);
 
+/* ---- batched layered circuits ------------------------------------------------ */
+
+/* Many independent circuits of uneven size, forward pass + energy + adjoint
+ * gradients in one replayed hipGraph.  No single reference function: it
+ * batches what AdjointJacobianLQubit.hpp:212-336 computes per circuit for one
+ * observable (K = 1), for the layered-ansatz family
+ *   for each of depth[j] layers: for w in 0..n-1: the rotations named by
+ *   `pattern` (e.g. "YZ" = RY then RZ, each with its own angle), then the
+ *   CNOT entangler (0 none, 1 ladder i->i+1, 2 ring i->(i+1) mod n).
+ * prog/coeffs: ncirc observables in the program encoding (Pauli words or
+ * Hamiltonians of Pauli words).  tile_width 0 lets the autotuner choose.
+ * Angles and gradients of circuit j are laid out layer-major, then wire, then
+ * pattern letter -- the tape order of the reference's operations. */
+typedef struct pl_batch *pl_batch_t;
+int pl_batch_create(int dtype, int ncirc, const int *nqubits, const int *depth,
+                    const char *pattern, int entangler, const int *prog,
+                    const double *coeffs, int tile_width, pl_batch_t *out);
+/* energies[ncirc]; grads[total angles] (may be NULL) */
+int pl_batch_run(pl_batch_t b, const double *params, double *energies,
+                 double *grads);
+int pl_batch_info(pl_batch_t b, int *tile_width, int *launches,
+                  int64_t *nparams);
+int pl_batch_destroy(pl_batch_t b);
+
 /* ---- timing ------------------------------------------------------------------ */
 
 /* HIP events recorded on the handle's stream around the calls in between;

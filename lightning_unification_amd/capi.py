@@ -26,7 +26,8 @@ EXPORTS = [
     "pl_sv_apply_ops", "pl_sv_apply_matrix", "pl_sv_apply_generator", "pl_expval", "pl_var",
     "pl_expval_named", "pl_var_named", "pl_expval_matrix", "pl_probs", "pl_sample",
     "pl_generate_samples", "pl_adjoint_jacobian", "pl_timer_start", "pl_timer_stop",
-    "pl_event_record", "pl_event_elapsed", "pl_sv_apply_observable",
+    "pl_event_record", "pl_event_elapsed", "pl_sv_apply_observable", "pl_batch_create",
+    "pl_batch_run", "pl_batch_info", "pl_batch_destroy",
 ]
 
 _lib = None
@@ -51,6 +52,11 @@ def load():
                                             ctypes.c_void_p]
         lib.pl_sv_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         lib.pl_sv_download.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        lib.pl_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        lib.pl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib = lib
     return _lib
 
@@ -252,3 +258,51 @@ class DeviceStateVector:
         ms = ctypes.c_double()
         _check(load().pl_event_elapsed(self.h, int(a), int(b), ctypes.byref(ms)))
         return ms.value
+
+
+class BatchPlan:
+    """Batched layered-ansatz circuits (pl_batch_*): one hipGraph replay gives
+    every circuit's energy and gradient."""
+
+    def __init__(self, nqubits, depths, pattern, entangler, observables, dtype=np.complex128,
+                 tile_width=0):
+        lib = load()
+        self.nqubits = [int(x) for x in nqubits]
+        self.depths = [int(x) for x in depths]
+        if len(self.nqubits) != len(self.depths) or len(observables) != len(self.nqubits):
+            raise ValueError("need one qubit count, depth and observable per circuit")
+        from .circuits import ENTANGLERS
+        ent = ENTANGLERS[entangler] if isinstance(entangler, str) else int(entangler)
+        prog, coeffs, _ = encode_observables(observables)
+        code = PL_C128 if np.dtype(dtype) == np.complex128 else PL_C64
+        h = ctypes.c_void_p()
+        _check(lib.pl_batch_create(code, len(self.nqubits), _p(_i32(self.nqubits)),
+                                   _p(_i32(self.depths)), pattern.encode(), ent, _p(prog),
+                                   _p(coeffs), int(tile_width), ctypes.byref(h)))
+        self.h = h
+        w, nl, npar = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib.pl_batch_info(self.h, ctypes.byref(w), ctypes.byref(nl), ctypes.byref(npar)))
+        self.tile_width, self.launches, self.nparams = w.value, nl.value, npar.value
+        self.pattern = pattern
+        self.param_offsets = np.concatenate(
+            [[0], np.cumsum([d * n * len(pattern) for n, d in zip(self.nqubits, self.depths)])])
+
+    def run(self, params, grads=True):
+        p = _f64(params)
+        if p.size < self.nparams:
+            raise ValueError("need one angle per rotation")
+        e = np.zeros(len(self.nqubits))
+        g = np.zeros(max(1, self.nparams)) if grads else None
+        _check(load().pl_batch_run(self.h, _p(p), _p(e), _p(g) if grads else None))
+        return (e, g[: self.nparams]) if grads else e
+
+    def split(self, flat):
+        o = self.param_offsets
+        return [flat[o[j]:o[j + 1]] for j in range(len(self.nqubits))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pl_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close

@@ -231,6 +231,28 @@ def sample_uniforms(m, seed=1002, stream=9):
     return Rng(seed, stream).uniform(m)
 
 
+ENTANGLERS = {"none": 0, "ladder": 1, "ring": 2}
+
+
+def layered_tape(n, depth, pattern, entangler, params):
+    """The layered-ansatz family in tape order: per layer, for each wire the
+    rotations of `pattern` (e.g. "YZ" = RY then RZ) with consecutive angles
+    from `params`, then CNOT(i, i+1) for a ladder / CNOT(i, (i+1) mod n) for
+    a ring."""
+    tape = Tape()
+    k = 0
+    for _ in range(depth):
+        for w in range(n):
+            for a in pattern:
+                tape.add("R" + a, [w], [params[k]])
+                k += 1
+        if entangler != "none" and n > 1:
+            last = n if entangler == "ring" else n - 1
+            for i in range(last):
+                tape.add("CNOT", [i, (i + 1) % n])
+    return tape
+
+
 def hea(n, depth, rng):
     """Hardware-efficient ansatz: RY, RZ on every wire, then a CNOT ring."""
     tape = Tape()
