@@ -81,6 +81,74 @@ def cpu_baseline(tape, n, budget_s=20.0):
                       f"{t_total:.1f} s single-threaded"}
 
 
+def bench_batch(count, plain_budget_s=8.0, reps=3):
+    """Batching form: `count` HEA circuits (8..20 qubits, depth 4..12, RY+RZ
+    + CNOT ring, periodic TFIM each), energy + full gradient for every circuit.
+    Batched: one hipGraph replay of pl_batch_run per step (all circuits).
+    Plain: the same circuits one at a time through the single-state path
+    (reset, apply_ops, expval, pl_adjoint_jacobian), on a bounded sample.
+    grads/s := complete (E, grad E) circuits per second."""
+    from lightning_unification_amd import capi
+    from lightning_unification_amd.circuits import batch_shapes, layered_tape, tfim
+
+    ns, ds, hs, params = batch_shapes(count)
+    obs = [tfim(n, 1.0, h) for n, h in zip(ns, hs)]
+    out = {"circuits": count, "params": int(params.size),
+           "qubits": [int(min(ns)), int(max(ns))], "depths": [int(min(ds)), int(max(ds))]}
+    res = {}
+    for name, dt in (("c128", np.complex128), ("c64", np.complex64)):
+        t0 = time.perf_counter()
+        plan = capi.BatchPlan(ns, ds, "YZ", "ring", obs, dtype=dt)
+        t_setup = time.perf_counter() - t0
+        e, g = plan.run(params)  # warm replay
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e, g = plan.run(params)
+        dt_s = (time.perf_counter() - t0) / reps
+        res[name] = (e, g)
+        out[name] = {"grads_per_s": round(count / dt_s, 1), "ms_per_batch": round(dt_s * 1e3, 2),
+                     "tile_width": plan.tile_width, "graph_launches": plan.launches,
+                     "setup_s": round(t_setup, 2)}
+        log(f"batched {name}: {dt_s*1e3:.1f} ms per {count} circuits (W={plan.tile_width})")
+        plan.close()
+    e128, g128 = res["c128"]
+    e64, g64 = res["c64"]
+    out["c64_vs_c128"] = {"max_abs_energy_err": float(np.max(np.abs(e64 - e128))),
+                          "max_abs_grad_err": float(np.max(np.abs(g64 - g128)))}
+
+    # plain: the single-circuit API, circuit by circuit over a seeded random
+    # sample of the same batch (an unbiased mix), until the budget is spent
+    from lightning_unification_amd.circuits import Rng
+    offs = np.concatenate([[0], np.cumsum([2 * n * d for n, d in zip(ns, ds)])])
+    done, t_plain, max_diff = 0, 0.0, 0.0
+    svs = {}
+    for j in Rng(5).shuffle(range(count)):
+        n, d = ns[j], ds[j]
+        p = params[offs[j]:offs[j + 1]]
+        tape = layered_tape(n, d, "YZ", "ring", p)
+        sv = svs.get(n) or svs.setdefault(n, capi.DeviceStateVector(n, np.complex128))
+        t0 = time.perf_counter()
+        sv.reset()
+        sv.apply_tape(tape)
+        ej = sv.expval(obs[j])
+        jac = sv.adjoint_jacobian(tape, [obs[j]], list(range(p.size)))
+        t_plain += time.perf_counter() - t0
+        max_diff = max(max_diff, abs(ej - e128[j]),
+                       float(np.max(np.abs(jac[0] - g128[offs[j]:offs[j + 1]]))))
+        done += 1
+        if t_plain > plain_budget_s:
+            break
+    for sv in svs.values():
+        sv.close()
+    plain_gps = done / t_plain
+    out["plain_c128"] = {"grads_per_s": round(plain_gps, 1), "sample_circuits": done,
+                         "sample_s": round(t_plain, 2), "max_abs_diff_vs_batched": max_diff}
+    out["speedup_c128"] = round(out["c128"]["grads_per_s"] / plain_gps, 1)
+    out["speedup_c64"] = round(out["c64"]["grads_per_s"] / plain_gps, 1)
+    log(f"plain: {done} circuits in {t_plain:.1f} s; speedup c128 {out['speedup_c128']}x")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,7 +157,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--shots", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["gates", "copy"], default="gates",
+    ap.add_argument("--batch", type=int, default=4096,
+                    help="circuits of the batching workload (0 skips it)")
+    ap.add_argument("--mode", choices=["gates", "copy", "batch"], default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
@@ -106,6 +176,9 @@ def main():
             b.copy_from(a)
         b.synchronize()
         print(json.dumps({"mode": "copy", "bytes_per_copy": (1 << 28) * 16, "copies": 3}))
+        return
+    if args.mode == "batch":
+        print(json.dumps({"mode": "batch", "batched": bench_batch(args.batch)}))
         return
     n = args.qubits
     tape = random_layers(n, 50, seed=1002)
@@ -173,6 +246,7 @@ def main():
     log(f"<H> = {e:.12f} ({t_expval*1e3:.1f} ms), {len(idx)} samples ({t_sample*1e3:.1f} ms)")
     sv.close()
 
+    batched = bench_batch(args.batch) if args.batch > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(tape, n)
 
     out = {
@@ -201,6 +275,7 @@ def main():
         "per_gate_GBs": per_gate_gbs,
         "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},
         "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
+        "batched": batched,
     }
     print(json.dumps(out))
 

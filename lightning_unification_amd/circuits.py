@@ -289,3 +289,19 @@ def batch_circuits(count=4096, seed=1004, nmin=8, nmax=20, dmin=4, dmax=12):
         h = 0.5 + float(rng.uniform(1)[0])
         out.append((n, hea(n, d, rng), tfim(n, 1.0, h)))
     return out
+
+
+def batch_shapes(count=4096, seed=1004, nmin=8, nmax=20, dmin=4, dmax=12):
+    """The same workload as batch_circuits without building tapes:
+    -> (qubits[], depths[], fields h[], flat angles in tape order).  Angles
+    are the draws hea() makes (RY then RZ per wire per layer)."""
+    ns, ds, hs, ps = [], [], [], []
+    for j in range(count):
+        rng = Rng(seed, j)
+        n = nmin + rng.integer(nmax - nmin + 1)
+        d = dmin + rng.integer(dmax - dmin + 1)
+        hs.append(0.5 + float(rng.uniform(1)[0]))
+        ns.append(n)
+        ds.append(d)
+        ps.append(2.0 * math.pi * rng.uniform(2 * n * d))
+    return ns, ds, hs, np.concatenate(ps)

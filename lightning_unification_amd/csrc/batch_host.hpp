@@ -197,14 +197,9 @@ template <class T> struct BatchPlan final : BatchPlanBase {
     }
 
     static int stages(int n, int w) { return n <= w ? 1 : 1 + (n - w + (w - 4)) / (w - 3); }
-    static int stage_T(int n, int w, int s) {
-        if (s == 0) {
-            return n < w ? n : w;
-        }
-        const int alo = w + (s - 1) * (w - 3);
-        const int ahi = std::min(alo + (w - 3), n);
-        return 3 + (ahi - alo);
-    }
+    // tile width of stage s (stage_bits in kernels_batch.hpp): later stages
+    // always use full W-bit tiles
+    static int stage_T(int n, int w, int s) { return s == 0 ? std::min(n, w) : w; }
 
     // Build circuit tables and the launch sequence for tile width w.
     void configure(int w) {
@@ -262,6 +257,10 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                 }
             }
             prefixes.push_back(tot);
+            if (tot * kBlock > 0xFFFFFFFFull) { // work-items per launch dimension
+                throw InvalidArg("batch too large for tile width " + std::to_string(w) +
+                                 ": one launch would exceed 2^32 work-items");
+            }
             if (cnt == 0) {
                 lists.resize(lo);
                 prefixes.resize(po);
@@ -356,8 +355,14 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                                    dparams, dgrads);
                 break;
             }
+            const hipError_t err = hipGetLastError();
+            if (err != hipSuccess) {
+                throw DeviceError("batch launch (kind " + std::to_string(static_cast<int>(ls.kind)) +
+                                  ", layer " + std::to_string(ls.layer) + ", stage " +
+                                  std::to_string(ls.stage) + ", " + std::to_string(ls.blocks) +
+                                  " blocks, W " + std::to_string(W) + "): " + hipGetErrorString(err));
+            }
         }
-        HIP_OK(hipGetLastError());
     }
 
     void capture() {
@@ -393,7 +398,11 @@ template <class T> struct BatchPlan final : BatchPlanBase {
             if (w > nmax && w != 8) {
                 break;
             }
-            configure(w);
+            try {
+                configure(w);
+            } catch (const InvalidArg &) { // this width needs too many work-items
+                continue;
+            }
             run(p.data(), e.data(), g.data()); // warm-up
             const auto t0 = std::chrono::steady_clock::now();
             run(p.data(), e.data(), g.data());

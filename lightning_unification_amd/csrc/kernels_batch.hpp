@@ -60,9 +60,12 @@ __device__ __forceinline__ int find_circ(const BLaunch &L, uint64_t b, uint64_t 
 }
 
 // Tile geometry of rotation stage s with tile width W: stage 0 spans bits
-// [0, min(n, W)); stage s > 0 spans bits {0,1,2} (coalescing only) plus the
-// target bits [W + (s-1)(W-3), ...).  Returns T, fills bits[] (ascending)
-// and the active target range [alo, ahi).
+// [0, min(n, W)) and rotates all of them; stage s > 0 (only when n > W)
+// rotates the target bits [alo, ahi) = [W + (s-1)(W-3), +W-3) clipped to n,
+// inside a full W-bit tile: bits {0,1,2} (one 128-byte line, for coalescing)
+// plus W-3 consecutive high bits ending no later than n, so the last stage's
+// tile also holds some bits of the stage before it, left untouched.  Returns
+// T, fills bits[] (ascending) and [alo, ahi).
 __device__ __forceinline__ int stage_bits(int n, int W, int s, int *bits, int &alo, int &ahi) {
     if (s == 0) {
         const int T = n < W ? n : W;
@@ -75,11 +78,12 @@ __device__ __forceinline__ int stage_bits(int n, int W, int s, int *bits, int &a
     }
     alo = W + (s - 1) * (W - 3);
     ahi = alo + (W - 3) < n ? alo + (W - 3) : n;
+    const int hlo = alo < n - (W - 3) ? alo : n - (W - 3);
     int T = 0;
     for (int k = 0; k < 3; k++) {
         bits[T++] = k;
     }
-    for (int b = alo; b < ahi; b++) {
+    for (int b = hlo; b < hlo + (W - 3); b++) {
         bits[T++] = b;
     }
     return T;
