@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", type=int, default=4096,
                     help="circuits of the batching workload (0 skips it)")
+    ap.add_argument("--plain-s", type=float, default=8.0,
+                    help="seconds of the per-circuit path timed for the batching comparison")
     ap.add_argument("--mode", choices=["gates", "copy", "batch"], default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
@@ -178,7 +180,7 @@ def main():
         print(json.dumps({"mode": "copy", "bytes_per_copy": (1 << 28) * 16, "copies": 3}))
         return
     if args.mode == "batch":
-        print(json.dumps({"mode": "batch", "batched": bench_batch(args.batch)}))
+        print(json.dumps({"mode": "batch", "batched": bench_batch(args.batch, args.plain_s)}))
         return
     n = args.qubits
     tape = random_layers(n, 50, seed=1002)
@@ -246,7 +248,7 @@ def main():
     log(f"<H> = {e:.12f} ({t_expval*1e3:.1f} ms), {len(idx)} samples ({t_sample*1e3:.1f} ms)")
     sv.close()
 
-    batched = bench_batch(args.batch) if args.batch > 0 else None
+    batched = bench_batch(args.batch, args.plain_s) if args.batch > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(tape, n)
 
     out = {

@@ -5,7 +5,7 @@
  * C++ classes in lightning_unification_amd/csrc/host/ (StateVector,
  * Measurements, Observables, AdjointJacobian -- the C++ API surface of
  * Lightning-Qubit) and the pybind11 module `lightning_qubit_ops` sit on top of
- * it; a ctypes / cgo / N-API binding can sit on it directly (INTEGRATION.md).
+ * it; a ctypes / cgo / N-API binding can sit on it directly.
  *
  * Conventions
  *  - Every function returns 0 on success.  On failure it returns a nonzero

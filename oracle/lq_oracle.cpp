@@ -5,8 +5,7 @@
  * cpu_baseline leg of bench.py load this library, as the checker; the product
  * package never links or calls it (see oracle_api.h for the interface).
  *
- * What it restates, and what pins each part (the full table lives in the
- * repository's design notes):
+ * What it restates, and what pins each part:
  *   gate kernels      GateImplementationsLM.hpp (default kernel for all gates
  *                     but Toffoli/CSWAP/DoubleExcitation*, which follow
  *                     GateImplementationsPI.hpp; AssignKernelMap_Default.cpp:41-231)
