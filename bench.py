@@ -149,6 +149,65 @@ def bench_batch(count, plain_budget_s=8.0, reps=3):
     return out
 
 
+def bench_precision(n_big=33, n_ref=30):
+    """Precision form: the same seeded 50-layer gate mix, consecutive gates
+    fused into 5-qubit 32x32 blocks applied to a complex64 state on FP32
+    matrix cores, renormalised in FP64 after every block.
+    Error reference at n_ref qubits: the fused complex64 run against the
+    unfused complex128 run of the same circuit (|d<H>|, ||d psi||_2, norm
+    drift without renormalisation).  Throughput at n_big qubits (64 GiB)."""
+    from lightning_unification_amd import capi
+    out = {}
+    # error reference
+    tape = random_layers(n_ref, 50, seed=1002)
+    ham = random_hamiltonian(n_ref, 100, seed=1002)
+    s128 = capi.DeviceStateVector(n_ref, np.complex128)
+    s128.apply_tape(tape)
+    e128 = s128.expval(ham)
+    s64 = capi.DeviceStateVector(n_ref, np.complex64)
+    nb = s64.apply_tape_fused(tape, 5, renorm=True)
+    e64 = s64.expval(ham)
+    dist = s64.distance(s128)
+    s64.reset()
+    s64.apply_tape_fused(tape, 5, renorm=False)
+    drift = abs(s64.norm2() - 1.0)
+    s64.close()
+    s128.close()
+    out["error_vs_c128"] = {"qubits": n_ref, "blocks": nb, "abs_dH": abs(e64 - e128),
+                            "H_c128": e128, "state_l2": dist,
+                            "norm_drift_without_renorm": drift}
+    log(f"precision n={n_ref}: |dH| {abs(e64 - e128):.2e}, |dpsi| {dist:.2e}, drift {drift:.2e}")
+
+    # throughput at n_big
+    warm = capi.DeviceStateVector(14, np.complex64)
+    warm.apply_tape_fused(random_layers(14, 2, seed=1), 5, renorm=True)
+    warm.close()
+    tape = random_layers(n_big, 50, seed=1002)
+    ham = random_hamiltonian(n_big, 100, seed=1002)
+    sv = capi.DeviceStateVector(n_big, np.complex64)
+    sv.synchronize()
+    sv.timer_start()
+    nb = sv.apply_tape_fused(tape, 5, renorm=True)
+    ms = sv.timer_stop()
+    S = (1 << n_big) * 8
+    gate_bytes = sum(2 * touched_fraction(op[0]) * S for op in tape.ops)
+    block_bytes = nb * 2 * S  # every block reads and writes the state once
+    flops = nb * (1 << (n_big - 5)) * 8 * 32 * 32
+    t0 = time.perf_counter()
+    e = sv.expval(ham)
+    t_h = time.perf_counter() - t0
+    norm = sv.norm2()
+    sv.close()
+    out["c64_fused"] = {"qubits": n_big, "gates": len(tape), "blocks": nb, "ms": round(ms, 1),
+                        "block_GBs": round(block_bytes / (ms / 1e3) / 1e9, 1),
+                        "gate_equivalent_GBs": round(gate_bytes / (ms / 1e3) / 1e9, 1),
+                        "mfma_TFLOPs": round(flops / (ms / 1e3) / 1e12, 2),
+                        "H": e, "H_ms": round(t_h * 1e3, 1), "norm": norm}
+    log(f"precision n={n_big}: {nb} blocks in {ms:.0f} ms, "
+        f"{block_bytes / (ms / 1e3) / 1e9:.0f} GB/s, {flops / (ms / 1e3) / 1e12:.1f} TFLOP/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,7 +220,9 @@ def main():
                     help="circuits of the batching workload (0 skips it)")
     ap.add_argument("--plain-s", type=float, default=8.0,
                     help="seconds of the per-circuit path timed for the batching comparison")
-    ap.add_argument("--mode", choices=["gates", "copy", "batch"], default="gates",
+    ap.add_argument("--precision", type=int, default=33,
+                    help="qubits of the complex64 fused-block run (0 skips the precision form)")
+    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision"], default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
@@ -181,6 +242,9 @@ def main():
         return
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", "batched": bench_batch(args.batch, args.plain_s)}))
+        return
+    if args.mode == "precision":
+        print(json.dumps({"mode": "precision", "precision": bench_precision(args.precision)}))
         return
     n = args.qubits
     tape = random_layers(n, 50, seed=1002)
@@ -249,6 +313,7 @@ def main():
     sv.close()
 
     batched = bench_batch(args.batch, args.plain_s) if args.batch > 0 else None
+    precision = bench_precision(args.precision) if args.precision > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(tape, n)
 
     out = {
@@ -278,6 +343,7 @@ def main():
         "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},
         "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
         "batched": batched,
+        "precision": precision,
     }
     print(json.dumps(out))
 

@@ -158,6 +158,41 @@ int pl_adjoint_jacobian(pl_sv_t sv, int nops, const int *ids,
                         const double *coeffs, const double *mats, int ntp,
                         const int *tp, double *jac);
 
+/* ---- precision path: gate fusion ------------------------------------------- */
+
+enum { PL_FUSE_RENORM = 1, PL_FUSE_NO_MFMA = 2 };
+
+/* The operations of StateVectorBase::applyOperations (StateVectorBase.hpp:84-120),
+ * with consecutive operations whose wires together span at most max_wires
+ * (1..5) wires multiplied into one dense block first (greedy, in tape order).
+ * On complex64 states with n >= 12 every block is widened to 5 wires and
+ * applied on FP32 matrix cores; elsewhere blocks use the dense-matrix kernels.
+ * flags: PL_FUSE_RENORM renormalises to unit norm (FP64 norm; on the matrix-
+ * core path after every block), PL_FUSE_NO_MFMA forces the dense-matrix
+ * kernels.  Operations wider than max_wires are applied unfused.
+ * *nblocks (may be NULL) receives the number of blocks. */
+int pl_sv_apply_ops_fused(pl_sv_t sv, int nops, const int *ids, const int *nwires,
+                          const int *wires, const uint8_t *inverse,
+                          const int *nparams, const double *params, int max_wires,
+                          int flags, int *nblocks);
+/* The fusion plan of pl_sv_apply_ops_fused, computed on the host only (no
+ * device is touched): for block b, first_op[b], block_nops[b], nbits[b]
+ * (0 = one operation applied unfused), bits[8 b ..] ascending bit positions,
+ * and mats[2 * 1024 b ..] the 2^nbits x 2^nbits row-major matrix (interleaved
+ * double complex) in the block-local index whose bit m is bit position
+ * bits[8 b + m].  Blocks narrower than pad_to are widened with identity bits.
+ * Fails with PL_ERR_INVALID, and sets *nblocks, when capacity is too small. */
+int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
+                 const int *wires, const uint8_t *inverse, const int *nparams,
+                 const double *params, int max_wires, int pad_to, int capacity,
+                 int *nblocks, int *first_op, int *block_nops, int *nbits,
+                 int *bits, double *mats);
+/* <psi|psi> accumulated in FP64 (innerProdC(psi, psi), LinearAlgebra.hpp:193-213). */
+int pl_sv_norm2(pl_sv_t sv, double *out);
+/* ||a - b||_2 accumulated in FP64; a and b must have the same qubit count and
+ * may differ in precision (the error of a complex64 run against complex128). */
+int pl_sv_distance(pl_sv_t a, pl_sv_t b, double *out);
+
 /* ---- batched layered circuits ------------------------------------------------ */
 
 /* Many independent circuits of uneven size, forward pass + energy + adjoint

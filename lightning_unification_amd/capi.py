@@ -27,8 +27,11 @@ EXPORTS = [
     "pl_expval_named", "pl_var_named", "pl_expval_matrix", "pl_probs", "pl_sample",
     "pl_generate_samples", "pl_adjoint_jacobian", "pl_timer_start", "pl_timer_stop",
     "pl_event_record", "pl_event_elapsed", "pl_sv_apply_observable", "pl_batch_create",
-    "pl_batch_run", "pl_batch_info", "pl_batch_destroy",
+    "pl_batch_run", "pl_batch_info", "pl_batch_destroy", "pl_sv_apply_ops_fused", "pl_fuse_tape",
+    "pl_sv_norm2", "pl_sv_distance",
 ]
+
+PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
 
 _lib = None
 
@@ -165,6 +168,29 @@ class DeviceStateVector:
                                       _p(_i32(wires)), _p(inv if inv.size else np.zeros(1, np.uint8)),
                                       _p(_i32(npar)), _p(_f64(par))))
 
+    def apply_tape_fused(self, tape, max_wires=5, renorm=False, mfma=True):
+        """Apply a tape as fused dense blocks (pl_sv_apply_ops_fused); returns
+        the number of blocks."""
+        ids, nw, wires, inv, npar, par = tape.encode()
+        flags = (PL_FUSE_RENORM if renorm else 0) | (0 if mfma else PL_FUSE_NO_MFMA)
+        nb = ctypes.c_int()
+        _check(load().pl_sv_apply_ops_fused(
+            self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+            _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)), _p(_f64(par)),
+            int(max_wires), flags, ctypes.byref(nb)))
+        return nb.value
+
+    def norm2(self):
+        out = ctypes.c_double()
+        _check(load().pl_sv_norm2(self.h, ctypes.byref(out)))
+        return out.value
+
+    def distance(self, other):
+        """||self - other||_2 in FP64 (the precisions may differ)."""
+        out = ctypes.c_double()
+        _check(load().pl_sv_distance(self.h, other.h, ctypes.byref(out)))
+        return out.value
+
     def apply_matrix(self, matrix, wires, inverse=False):
         m, w = _cmat(matrix), _i32(wires)
         _check(load().pl_sv_apply_matrix(self.h, _p(m), _p(w), len(wires), int(inverse)))
@@ -258,6 +284,31 @@ class DeviceStateVector:
         ms = ctypes.c_double()
         _check(load().pl_event_elapsed(self.h, int(a), int(b), ctypes.byref(ms)))
         return ms.value
+
+
+def fuse_tape(num_qubits, tape, max_wires=5, pad_to=0):
+    """Host-only fusion plan (pl_fuse_tape): list of blocks
+    (first_op, nops, bits or None for an unfused op, matrix or None)."""
+    ids, nw, wires, inv, npar, par = tape.encode()
+    cap = max(1, len(ids))
+    nb = ctypes.c_int()
+    first, nops, nbits = (np.zeros(cap, np.int32) for _ in range(3))
+    bits = np.zeros(8 * cap, np.int32)
+    mats = np.zeros(2048 * cap, np.float64)
+    _check(load().pl_fuse_tape(int(num_qubits), len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+                               _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)),
+                               _p(_f64(par)), int(max_wires), int(pad_to), cap, ctypes.byref(nb),
+                               _p(first), _p(nops), _p(nbits), _p(bits), _p(mats)))
+    out = []
+    for b in range(nb.value):
+        k = int(nbits[b])
+        if k == 0:
+            out.append((int(first[b]), int(nops[b]), None, None))
+            continue
+        d = 1 << k
+        m = mats[2048 * b: 2048 * b + 2 * d * d].view(np.complex128).reshape(d, d).copy()
+        out.append((int(first[b]), int(nops[b]), bits[8 * b: 8 * b + k].tolist(), m))
+    return out
 
 
 class BatchPlan:

@@ -1,0 +1,215 @@
+// Fused 5-qubit blocks on a complex64 state with FP32 matrix cores.
+//
+// A fused block is a dense 32x32 complex matrix U on five bit positions.  For
+// every group of 32 amplitudes (the five bits varying, all others fixed) the
+// kernel computes psi' = U psi as the real product
+//     [Re psi'; Im psi'] = [[Ur, -Ui], [Ui, Ur]] [Re psi; Im psi]
+// with v_mfma_f32_32x32x2f32 (32x32 output, K = 2 per instruction).
+//
+// A workgroup (4 waves) owns a tile of 4096 amplitudes: the 5 target bits and
+// the 7 lowest other bits ("column" bits), i.e. 128 groups.  The tile moves
+// HBM -> LDS -> HBM once, with whole 128-byte lines whenever the column bits
+// include bits 0..3.  LDS holds the tile as [target index j][column c].  Wave
+// w multiplies U into columns 32w .. 32w+31: for each K pair (j, j+1) a lane
+// supplies A = U[row = lane % 32][j + lane / 32] and B = x[j + lane / 32][c],
+// and the products of Ur and Ui with Re x and Im x go to four independent
+// accumulators (so consecutive MFMAs do not wait on one another).
+//
+// FP64 is kept where the precision form needs it: each tile's sum of |psi'|^2
+// is accumulated in double and written to a per-tile partial; a one-block
+// kernel reduces the partials in a fixed order to the renormalisation factor
+// that the next block (or the final scale pass) multiplies in.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels_reduce.hpp"
+
+namespace plamd {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kF5Tile = 4096; // amplitudes per workgroup tile
+constexpr int kF5Cols = 128;  // groups per tile
+
+struct Fused5Geom {
+    int pos[12];          // tile bit positions, ascending
+    uint32_t lds_of[12];  // LDS index contribution of each tile bit
+};
+
+// Per-lane matrix operands, host-packed: lane l, step s (K pair 2s, 2s+1):
+// a[l][s] = {Ur[l % 32][2s + l / 32], Ui[l % 32][2s + l / 32]}.
+struct Fused5Mat {
+    float2 a[64][16];
+};
+
+__device__ __forceinline__ uint64_t f5_base(uint64_t q, const Fused5Geom &g) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const int p = g.pos[k];
+        q = ((q >> p) << (p + 1)) | (q & ((uint64_t{1} << p) - 1));
+    }
+    return q;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fused5_mfma(float2 *__restrict__ sv, uint64_t ntiles,
+                                                        const Fused5Geom geo,
+                                                        const Fused5Mat *__restrict__ mat,
+                                                        const double *__restrict__ scale,
+                                                        double *__restrict__ partial) {
+    __shared__ float2 tile[kF5Tile];
+    __shared__ double wsum[kBlock / 64];
+    const uint64_t q = xcd_block();
+    if (q >= ntiles) {
+        return; // uniform per workgroup
+    }
+    const uint64_t base = f5_base(q, geo);
+    const int tid = threadIdx.x;
+
+    // thread tid owns tile elements e = tid + 256 r; element e sets tile bit k
+    // when bit k of e is set (bits 0..7 from tid, 8..11 from r)
+    uint64_t a_lo = 0;
+    uint32_t l_lo = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if ((tid >> k) & 1) {
+            a_lo |= uint64_t{1} << geo.pos[k];
+            l_lo |= geo.lds_of[k];
+        }
+    }
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint64_t a = base | a_lo;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if ((r >> k) & 1) {
+                a |= uint64_t{1} << geo.pos[8 + k];
+            }
+        }
+        v[r] = ld_nt<float>(sv + a);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint32_t l = l_lo;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if ((r >> k) & 1) {
+                l |= geo.lds_of[8 + k];
+            }
+        }
+        tile[l] = v[r];
+    }
+    __syncthreads();
+
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col = wave * 32 + (lane & 31), half = lane >> 5;
+    const float s = scale ? static_cast<float>(*scale) : 1.0f;
+    floatx16 rr = {}, ri = {}, ir = {}, ii = {}; // Ur.Re, Ui.Im, Ui.Re, Ur.Im
+#pragma unroll
+    for (int st = 0; st < 16; st++) {
+        const float2 a = mat->a[lane][st];
+        const float2 x = tile[(2 * st + half) * kF5Cols + col];
+        const float ur = a.x * s, ui = a.y * s;
+        rr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.x, rr, 0, 0, 0);
+        ri = __builtin_amdgcn_mfma_f32_32x32x2f32(ui, x.y, ri, 0, 0, 0);
+        ir = __builtin_amdgcn_mfma_f32_32x32x2f32(ui, x.x, ir, 0, 0, 0);
+        ii = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.y, ii, 0, 0, 0);
+    }
+    // every lane of this wave has read its operands from columns it alone
+    // writes below; the wave's LDS accesses complete in program order
+    double nrm = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int row = 8 * (i >> 2) + 4 * half + (i & 3);
+        float2 y;
+        y.x = rr[i] - ri[i];
+        y.y = ir[i] + ii[i];
+        tile[row * kF5Cols + col] = y;
+        nrm += static_cast<double>(y.x) * y.x + static_cast<double>(y.y) * y.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint64_t a = base | a_lo;
+        uint32_t l = l_lo;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if ((r >> k) & 1) {
+                a |= uint64_t{1} << geo.pos[8 + k];
+                l |= geo.lds_of[8 + k];
+            }
+        }
+        st_nt<float>(sv + a, tile[l]);
+    }
+    if (partial) {
+        nrm = wave_sum(nrm);
+        if (lane == 0) {
+            wsum[wave] = nrm;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            partial[q] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+        }
+    }
+}
+
+// scale_out = 1 / sqrt(sum of partial[0 .. count)), fixed summation order
+__global__ __launch_bounds__(1024) void k_norm_scale(const double *__restrict__ partial,
+                                                     uint64_t count, double *__restrict__ scale_out) {
+    __shared__ double ws[16];
+    double acc = 0;
+    for (uint64_t i = threadIdx.x; i < count; i += 1024) {
+        acc += partial[i];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) {
+        ws[threadIdx.x >> 6] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < 16; w++) {
+            t += ws[w];
+        }
+        scale_out[0] = t > 0 ? 1.0 / sqrt(t) : 1.0;
+    }
+}
+
+// psi *= scale (the last renormalisation of a fused run)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scale(v2<T> *__restrict__ sv, uint64_t len,
+                                                  const double *__restrict__ scale) {
+    const T s = static_cast<T>(*scale);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        v2<T> a = ld_nt<T>(sv + i);
+        a.x *= s;
+        a.y *= s;
+        st_nt<T>(sv + i, a);
+    }
+}
+
+// partial[b] = sum over the block's i of |a_i - b_i|^2 and |a_i|^2, FP64,
+// for states of any two precisions
+template <class A, class B>
+__global__ __launch_bounds__(kBlock) void k_diff_norm(const v2<A> *__restrict__ a,
+                                                      const v2<B> *__restrict__ b, uint64_t len,
+                                                      double2 *__restrict__ partial) {
+    double2 acc{0, 0};
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const v2<A> x = a[i];
+        const v2<B> y = b[i];
+        const double dr = double(x.x) - double(y.x), di = double(x.y) - double(y.y);
+        acc.x += dr * dr + di * di;
+        acc.y += double(x.x) * double(x.x) + double(x.y) * double(x.y);
+    }
+    const double2 r = block_sum2(acc);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = r;
+    }
+}
+
+} // namespace plamd

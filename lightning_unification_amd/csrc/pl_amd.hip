@@ -22,6 +22,8 @@
 #include "kernels_gate.hpp"
 #include "kernels_reduce.hpp"
 #include "kernels_sample.hpp"
+#include "kernels_mfma.hpp"
+#include "fuse.hpp"
 #include "lower.hpp"
 
 namespace plamd {
@@ -366,6 +368,16 @@ template <class T> struct Engine {
     }
 
     template <int M> void launch_big(const GroupOp<T> &g, V *sv) {
+        std::vector<dcx<T>> m(M * M);
+        for (int j = 0; j < M * M; j++) {
+            m[j] = {g.mat[j].real(), g.mat[j].imag()};
+        }
+        launch_big_dev<M>(g, sv, static_cast<const dcx<T> *>(stage(h, m.data(), m.size() * sizeof(dcx<T>))));
+        HIP_OK(hipStreamSynchronize(h->stream)); // staging is reused by the next call
+    }
+
+    // dmat: the M x M matrix already in device memory
+    template <int M> void launch_big_dev(const GroupOp<T> &g, V *sv, const dcx<T> *dmat) {
         BigOp<T, M> op{};
         op.geo.fixed = g.fixed;
         op.geo.nb = g.nb;
@@ -375,16 +387,11 @@ template <class T> struct Engine {
         for (int j = 0; j < M; j++) {
             op.off[j] = g.off[j];
         }
-        std::vector<dcx<T>> m(M * M);
-        for (int j = 0; j < M * M; j++) {
-            m[j] = {g.mat[j].real(), g.mat[j].imag()};
-        }
-        op.mat = static_cast<const dcx<T> *>(stage(h, m.data(), m.size() * sizeof(dcx<T>)));
+        op.mat = dmat;
         const uint64_t nwork = len() >> g.nb;
         hipLaunchKernelGGL((k_group_big<T, M>), dim3(grid_for(nwork)), dim3(kBlock), 0, h->stream,
                            sv, nwork, op);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipStreamSynchronize(h->stream)); // staging is reused by the next call
     }
 
     void launch_huge(const GroupOp<T> &g, V *sv) {
@@ -468,6 +475,155 @@ template <class T> struct Engine {
         const auto g = lower_generator<T>(n(), id, w, nw, false, &s);
         apply(g, sv);
         return static_cast<double>(s);
+    }
+
+    // ---- precision path: fused blocks -----------------------------------------
+    static constexpr bool kMfma = std::is_same_v<T, float>;
+    bool mfma_ok(const FusedBlock &b, int flags) const {
+        return kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 12 && b.fused && b.bits.size() == 5;
+    }
+
+    // tile geometry of k_fused5_mfma: 5 target bits + the 7 lowest other bits
+    Fused5Geom fused5_geom(const std::vector<int> &tb) const {
+        std::vector<std::pair<int, uint32_t>> all;
+        for (int m = 0; m < 5; m++) {
+            all.emplace_back(tb[m], uint32_t{kF5Cols} << m);
+        }
+        for (int p = 0, m = 0; m < 7; p++) {
+            if (std::find(tb.begin(), tb.end(), p) == tb.end()) {
+                all.emplace_back(p, 1U << m);
+                m++;
+            }
+        }
+        std::sort(all.begin(), all.end());
+        Fused5Geom g{};
+        for (int k = 0; k < 12; k++) {
+            g.pos[k] = all[k].first;
+            g.lds_of[k] = all[k].second;
+        }
+        return g;
+    }
+
+    // Applies a tape as fused blocks; returns the number of blocks.
+    int apply_fused(V *sv, const TapeView &t, int max_wires, int flags, std::vector<FusedBlock> *keep) {
+        const bool probe_mfma = kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 12;
+        auto blocks = fuse_tape(n(), t, max_wires, probe_mfma ? 5 : 0);
+        const bool renorm = flags & PL_FUSE_RENORM;
+
+        // one device buffer for every block matrix (no staging reuse, no sync per block)
+        std::vector<size_t> moff(blocks.size(), 0);
+        size_t mbytes = 0;
+        for (size_t i = 0; i < blocks.size(); i++) {
+            const auto &b = blocks[i];
+            moff[i] = mbytes;
+            if (!b.fused) {
+                continue;
+            }
+            if (mfma_ok(b, flags)) {
+                mbytes += sizeof(Fused5Mat);
+            } else if (b.U.size() >= 64) {
+                mbytes += b.U.size() * sizeof(dcx<T>);
+            }
+            mbytes = (mbytes + 255) & ~size_t{255};
+        }
+        const uint64_t ntiles = n() >= 12 ? (len() >> 12) : 1;
+        const size_t tail = ntiles * sizeof(double) + 256;
+        std::vector<unsigned char> host(mbytes + tail, 0);
+        for (size_t i = 0; i < blocks.size(); i++) {
+            const auto &b = blocks[i];
+            if (!b.fused) {
+                continue;
+            }
+            if (mfma_ok(b, flags)) {
+                auto *fm = reinterpret_cast<Fused5Mat *>(host.data() + moff[i]);
+                for (int l = 0; l < 64; l++) {
+                    for (int st = 0; st < 16; st++) {
+                        const auto u = b.U[(l & 31) * 32 + 2 * st + (l >> 5)];
+                        fm->a[l][st] = float2{static_cast<float>(u.real()), static_cast<float>(u.imag())};
+                    }
+                }
+            } else if (b.U.size() >= 64) {
+                auto *m = reinterpret_cast<dcx<T> *>(host.data() + moff[i]);
+                for (size_t j = 0; j < b.U.size(); j++) {
+                    m[j] = {static_cast<T>(b.U[j].real()), static_cast<T>(b.U[j].imag())};
+                }
+            }
+        }
+        unsigned char *dev = nullptr;
+        HIP_OK(hipMalloc(&dev, host.size()));
+        struct Free {
+            unsigned char *p;
+            hipStream_t s;
+            ~Free() {
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(p);
+            }
+        } guard{dev, h->stream};
+        HIP_OK(hipMemcpyAsync(dev, host.data(), mbytes, hipMemcpyHostToDevice, h->stream));
+        auto *partial = reinterpret_cast<double *>(dev + mbytes);
+        auto *scale = partial + ntiles;
+        bool pending = false; // scale holds a factor not yet applied
+
+        std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
+        for (int i = 0; i < t.nops; i++) {
+            woff[i + 1] = woff[i] + t.nwires[i];
+            poff[i + 1] = poff[i] + t.nparams[i];
+        }
+        for (size_t i = 0; i < blocks.size(); i++) {
+            const auto &b = blocks[i];
+            if (!b.fused) {
+                const int o = b.first_op;
+                gate(sv, t.ids[o], t.wires + woff[o], t.nwires[o], t.inverse[o] != 0, t.params + poff[o],
+                     t.nparams[o]);
+                continue;
+            }
+            if (mfma_ok(b, flags)) {
+                if constexpr (kMfma) {
+                    hipLaunchKernelGGL(k_fused5_mfma, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0,
+                                       h->stream, sv, ntiles, fused5_geom(b.bits),
+                                       reinterpret_cast<const Fused5Mat *>(dev + moff[i]),
+                                       pending ? scale : nullptr, renorm ? partial : nullptr);
+                    HIP_OK(hipGetLastError());
+                    if (renorm) {
+                        hipLaunchKernelGGL(k_norm_scale, dim3(1), dim3(1024), 0, h->stream, partial, ntiles,
+                                           scale);
+                        HIP_OK(hipGetLastError());
+                        pending = true;
+                    } else {
+                        pending = false;
+                    }
+                }
+                continue;
+            }
+            std::vector<std::complex<T>> mt(b.U.size());
+            for (size_t j = 0; j < b.U.size(); j++) {
+                mt[j] = {static_cast<T>(b.U[j].real()), static_cast<T>(b.U[j].imag())};
+            }
+            const auto g = make_group<T>(b.bits, 0, block_offsets(b.bits), std::move(mt));
+            switch (g.M) {
+            case 8: launch_big_dev<8>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
+            case 16: launch_big_dev<16>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
+            case 32: launch_big_dev<32>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
+            default: apply(g, sv); break; // M <= 4: matrix in the kernel arguments
+            }
+        }
+        if (renorm) {
+            if (!pending) { // no MFMA block produced a norm: one FP64 norm pass
+                const double nrm = dot(sv, sv).real();
+                const double sc = nrm > 0 ? 1.0 / std::sqrt(nrm) : 1.0;
+                HIP_OK(hipMemcpyAsync(scale, &sc, sizeof(double), hipMemcpyHostToDevice, h->stream));
+                HIP_OK(hipStreamSynchronize(h->stream));
+            }
+            hipLaunchKernelGGL((k_scale<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream, sv, len(),
+                               scale);
+            HIP_OK(hipGetLastError());
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+        const int nb = static_cast<int>(blocks.size());
+        if (keep) {
+            *keep = std::move(blocks);
+        }
+        return nb;
     }
 
     // ---- reductions ----------------------------------------------------------
@@ -1228,6 +1384,96 @@ int pl_sv_apply_ops(pl_sv_t h, int nops, const int *ids, const int *nwires, cons
             wo += nwires[i];
             po += nparams[i];
         }
+    });
+}
+
+int pl_sv_apply_ops_fused(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
+                          const uint8_t *inverse, const int *nparams, const double *params,
+                          int max_wires, int flags, int *nblocks) {
+    return dispatch(h, [&](auto &E) {
+        if (nops < 0 || (nops > 0 && (!ids || !nwires || !wires || !inverse || !nparams))) {
+            throw InvalidArg("null tape argument");
+        }
+        const TapeView t{nops, ids, nwires, wires, inverse, nparams, params};
+        const int nb = E.apply_fused(E.psi(), t, max_wires, flags, nullptr);
+        if (nblocks) {
+            *nblocks = nb;
+        }
+    });
+}
+
+int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires, const int *wires,
+                 const uint8_t *inverse, const int *nparams, const double *params, int max_wires,
+                 int pad_to, int capacity, int *nblocks, int *first_op, int *block_nops, int *nbits,
+                 int *bits, double *mats) {
+    return guarded([&] {
+        if (num_qubits < 1 || num_qubits > 62 || nops < 0 || !nblocks) {
+            throw InvalidArg("invalid argument");
+        }
+        const TapeView t{nops, ids, nwires, wires, inverse, nparams, params};
+        const auto blocks = fuse_tape(num_qubits, t, max_wires, std::max(0, std::min(pad_to, 5)));
+        *nblocks = static_cast<int>(blocks.size());
+        if (*nblocks > capacity) {
+            throw InvalidArg("capacity too small for " + std::to_string(*nblocks) + " blocks");
+        }
+        for (size_t b = 0; b < blocks.size(); b++) {
+            const auto &B = blocks[b];
+            if (first_op) first_op[b] = B.first_op;
+            if (block_nops) block_nops[b] = B.nops;
+            if (nbits) nbits[b] = B.fused ? static_cast<int>(B.bits.size()) : 0;
+            for (size_t m = 0; bits && B.fused && m < B.bits.size(); m++) {
+                bits[8 * b + m] = B.bits[m];
+            }
+            for (size_t j = 0; mats && B.fused && j < B.U.size(); j++) {
+                mats[2048 * b + 2 * j] = B.U[j].real();
+                mats[2048 * b + 2 * j + 1] = B.U[j].imag();
+            }
+        }
+    });
+}
+
+int pl_sv_norm2(pl_sv_t h, double *out) {
+    return dispatch(h, [&](auto &E) {
+        if (!out) {
+            throw InvalidArg("null output");
+        }
+        *out = E.dot(E.psi(), E.psi()).real();
+    });
+}
+
+int pl_sv_distance(pl_sv_t a, pl_sv_t b, double *out) {
+    return guarded([&] {
+        if (!a || !b || !out) {
+            throw InvalidArg("null argument");
+        }
+        if (a->n != b->n) {
+            throw InvalidArg("states must have the same number of qubits");
+        }
+        HIP_OK(hipSetDevice(0));
+        HIP_OK(hipStreamSynchronize(b->stream)); // b's pending work, then run on a's stream
+        const unsigned grid = std::min<unsigned>(grid_for(a->len, 1, 4), kReduceBlocks);
+        auto *part = reinterpret_cast<double2 *>(reduce_buf(a, (kReduceBlocks + 1) * sizeof(double2)));
+        auto launch = [&](auto *pa, auto *pb) {
+            using A = std::remove_pointer_t<decltype(pa)>;
+            using B = std::remove_pointer_t<decltype(pb)>;
+            hipLaunchKernelGGL((k_diff_norm<typename A::value_type, typename B::value_type>), dim3(grid),
+                               dim3(kBlock), 0, a->stream, reinterpret_cast<const typename A::type *>(a->d),
+                               reinterpret_cast<const typename B::type *>(b->d), a->len, part);
+        };
+        struct D { using value_type = double; using type = v2<double>; };
+        struct F { using value_type = float; using type = v2<float>; };
+        if (a->dtype == PL_C128 && b->dtype == PL_C128) launch((D *)nullptr, (D *)nullptr);
+        else if (a->dtype == PL_C128) launch((D *)nullptr, (F *)nullptr);
+        else if (b->dtype == PL_C128) launch((F *)nullptr, (D *)nullptr);
+        else launch((F *)nullptr, (F *)nullptr);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, a->stream, part, (int)grid,
+                           part + kReduceBlocks);
+        HIP_OK(hipGetLastError());
+        double2 r;
+        HIP_OK(hipMemcpyAsync(&r, part + kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, a->stream));
+        HIP_OK(hipStreamSynchronize(a->stream));
+        *out = std::sqrt(r.x);
     });
 }
 
