@@ -49,6 +49,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs (scaled by a copy-kernel calibration), read from the
+    committed summary under profiles/; None when no summary covers it."""
+    pdir = os.path.join(ROOT, "profiles")
+    key = f"traffic_bytes_per_launch {kernel}:"
+    for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                if line.startswith(key):
+                    return int(line.split(":")[1]), f"profiles/{name}"
+    return None, None
+
+
 def cpu_baseline(tape, n, budget_s=20.0):
     """Time the reference LQ kernels (or the oracle) on the first gates of the
     same circuit at the same size; single-threaded like the reference's gate
@@ -294,6 +308,7 @@ def main():
         d["gates"].add(op[0])
     dom = max(by_kernel, key=lambda k: by_kernel[k]["ms"])
     dk = by_kernel[dom]
+    traffic, traffic_src = profiled_traffic(dom) if n == 30 else (None, None)
     achieved = dk["bytes"] / (dk["ms"] / 1e3) / 1e9
     per_gate_kind = {}
     for i, op in enumerate(tape.ops):
@@ -337,7 +352,7 @@ def main():
                      "launches": dk["launches"],
                      "avg_launch_ms": round(dk["ms"] / dk["launches"], 4),
                      "algorithmic_bytes_per_launch": round(dk["bytes"] / dk["launches"]),
-                     "traffic": None},
+                     "traffic": traffic, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "per_gate_GBs": per_gate_gbs,
         "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},

@@ -217,6 +217,36 @@ int pl_batch_info(pl_batch_t b, int *tile_width, int *launches,
                   int64_t *nparams);
 int pl_batch_destroy(pl_batch_t b);
 
+/* ---- learned surrogate -------------------------------------------------------- */
+
+/* A d0 -> d1 -> d2 -> d3 MLP (GELU hidden layers) trained and evaluated on the
+ * GPU with BF16 matrix cores and FP32 accumulation, FP32 master weights and
+ * Adam.  No reference counterpart: it screens optimiser candidates for the
+ * adjoint path (AdjointJacobianLQubit.hpp:212-336 stays the judge of every
+ * accepted step).  Parameters cross as FP32 in the order W1 [d0 x d1]
+ * row-major, b1, W2 [d1 x d2], b2, W3 [d2 x d3], b3; features as FP32 rows of
+ * d0 (rounded to BF16 on the device), targets and outputs as FP32 rows of d3.
+ * The loss is sum_j w_j (y_j - t_j)^2 averaged over the batch (w_j = 1 unless
+ * set). */
+typedef struct pl_mlp *pl_mlp_t;
+int pl_mlp_create(int d0, int d1, int d2, int d3, int max_batch, uint64_t seed,
+                  pl_mlp_t *out);
+int pl_mlp_destroy(pl_mlp_t m);
+int pl_mlp_num_params(pl_mlp_t m, int64_t *out);
+int pl_mlp_get_params(pl_mlp_t m, float *params);
+int pl_mlp_set_params(pl_mlp_t m, const float *params); /* also resets Adam */
+int pl_mlp_set_loss_weights(pl_mlp_t m, const float *w); /* d3 weights */
+/* y[n x d3] for features x[n x d0], n <= max_batch */
+int pl_mlp_predict(pl_mlp_t m, const float *x, int n, float *y);
+/* loss and its gradient (parameter layout) on one batch, n <= max_batch */
+int pl_mlp_gradients(pl_mlp_t m, const float *x, const float *t, int n,
+                     double *loss, float *grads);
+/* copy a training set to the device (features stored as BF16) */
+int pl_mlp_load_data(pl_mlp_t m, const float *x, const float *t, int64_t n);
+/* `steps` Adam steps over consecutive minibatches of the loaded data;
+ * *mean_loss (may be NULL) is the mean minibatch loss */
+int pl_mlp_train(pl_mlp_t m, int64_t steps, int batch, float lr, double *mean_loss);
+
 /* ---- timing ------------------------------------------------------------------ */
 
 /* HIP events recorded on the handle's stream around the calls in between;

@@ -28,7 +28,9 @@ EXPORTS = [
     "pl_generate_samples", "pl_adjoint_jacobian", "pl_timer_start", "pl_timer_stop",
     "pl_event_record", "pl_event_elapsed", "pl_sv_apply_observable", "pl_batch_create",
     "pl_batch_run", "pl_batch_info", "pl_batch_destroy", "pl_sv_apply_ops_fused", "pl_fuse_tape",
-    "pl_sv_norm2", "pl_sv_distance",
+    "pl_sv_norm2", "pl_sv_distance", "pl_mlp_create", "pl_mlp_destroy", "pl_mlp_num_params",
+    "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
+    "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -60,6 +62,10 @@ def load():
                                         ctypes.c_void_p, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]
         lib.pl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.pl_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
+        lib.pl_mlp_load_data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        lib.pl_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float,
+                                     ctypes.c_void_p]
         _lib = lib
     return _lib
 
@@ -357,3 +363,83 @@ class BatchPlan:
             self.h = None
 
     __del__ = close
+
+
+def _f32(x):
+    return np.ascontiguousarray(np.asarray(x, np.float32))
+
+
+class Mlp:
+    """The surrogate network (pl_mlp_*): d0 -> d1 -> d2 -> d3, GELU hidden
+    layers, BF16 matrix cores with FP32 accumulation, FP32 Adam."""
+
+    def __init__(self, dims=(192, 512, 512, 193), max_batch=1024, seed=1005):
+        self.dims = tuple(int(d) for d in dims)
+        self.max_batch = int(max_batch)
+        h = ctypes.c_void_p()
+        _check(load().pl_mlp_create(*self.dims, self.max_batch, int(seed), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_int64()
+        _check(load().pl_mlp_num_params(self.h, ctypes.byref(n)))
+        self.nparams = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().pl_mlp_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def params(self):
+        out = np.empty(self.nparams, np.float32)
+        _check(load().pl_mlp_get_params(self.h, _p(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        if p.size != self.nparams:
+            raise ValueError("wrong parameter count")
+        _check(load().pl_mlp_set_params(self.h, _p(p)))
+
+    def unpack(self, flat):
+        """-> [(W1, b1), (W2, b2), (W3, b3)] views of a flat parameter vector."""
+        d, out, o = self.dims, [], 0
+        for l in range(3):
+            w = flat[o:o + d[l] * d[l + 1]].reshape(d[l], d[l + 1])
+            o += d[l] * d[l + 1]
+            b = flat[o:o + d[l + 1]]
+            o += d[l + 1]
+            out.append((w, b))
+        return out
+
+    def set_loss_weights(self, w):
+        w = _f32(w)
+        if w.size != self.dims[3]:
+            raise ValueError("one weight per output")
+        _check(load().pl_mlp_set_loss_weights(self.h, _p(w)))
+
+    def predict(self, x):
+        x = _f32(x).reshape(-1, self.dims[0])
+        y = np.empty((x.shape[0], self.dims[3]), np.float32)
+        for r in range(0, x.shape[0], self.max_batch):
+            xr = np.ascontiguousarray(x[r:r + self.max_batch])
+            yr = np.empty((xr.shape[0], self.dims[3]), np.float32)
+            _check(load().pl_mlp_predict(self.h, _p(xr), xr.shape[0], _p(yr)))
+            y[r:r + xr.shape[0]] = yr
+        return y
+
+    def gradients(self, x, t):
+        x, t = _f32(x), _f32(t)
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_double()
+        _check(load().pl_mlp_gradients(self.h, _p(x), _p(t), x.shape[0], ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def load_data(self, x, t):
+        x, t = _f32(x), _f32(t)
+        _check(load().pl_mlp_load_data(self.h, _p(x), _p(t), x.shape[0]))
+
+    def train(self, steps, batch, lr):
+        loss = ctypes.c_double()
+        _check(load().pl_mlp_train(self.h, int(steps), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value

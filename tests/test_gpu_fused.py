@@ -50,7 +50,8 @@ def test_fused_c64_mfma(oracle, n):
     ref = capi.DeviceStateVector.from_array(exp)
     d = a.distance(ref)
     assert abs(d - np.linalg.norm(sa.astype(np.complex128) - exp)) < 1e-9
-    assert abs(a.norm2() - np.vdot(sa, sa).real) < 1e-9
+    s128 = sa.astype(np.complex128)
+    assert abs(a.norm2() - np.vdot(s128, s128).real) < 1e-9
     for s in (a, b, ref):
         s.close()
 
