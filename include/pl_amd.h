@@ -146,6 +146,32 @@ int pl_sample(pl_sv_t sv, const double *uniforms, int64_t nsamples,
 int pl_generate_samples(pl_sv_t sv, uint64_t seed, int64_t nsamples,
                         uint64_t *bits);
 
+/* Measurements::expval / var of a sparse Hamiltonian in CSR form
+ * (MeasurementsLQubit.hpp:183-214, 437-471; SparseLinAlg.hpp): row_map has
+ * 2^n + 1 entries (row_map[i] = non-zeros before row i), entries and values
+ * (interleaved complex) have nnz each.  One fused pass computes <psi|H psi>
+ * and ||H psi||^2 with FP64 accumulation; var = ||H psi||^2 - <psi|H psi>^2. */
+int pl_expval_sparse(pl_sv_t sv, const int64_t *row_map, int64_t row_map_size,
+                     const int64_t *entries, const double *values, int64_t nnz,
+                     double *out);
+int pl_var_sparse(pl_sv_t sv, const int64_t *row_map, int64_t row_map_size,
+                  const int64_t *entries, const double *values, int64_t nnz,
+                  double *out);
+/* Measurements::generate_samples_metropolis (MeasurementsLQubit.hpp:388-434,
+ * 613-640; TransitionKernels.hpp): Metropolis-Hastings samples of |psi|^2 in
+ * the generate_samples bit layout.  kernel 0 proposes uniformly among the
+ * indices with |psi_i| > epsilon of the state's precision; kernel 1 flips one
+ * uniformly chosen bit to a uniformly chosen value.  (The reference's
+ * kernel_factory builds kernel 0 for the name "Local" and kernel 1 for
+ * "NonZeroRandom"; the pybind layer keeps that mapping.)  nchains independent
+ * chains (0 = automatic) each start at index 0, discard num_burnin steps and
+ * record consecutive states, filling the output in chain order; SplitMix64
+ * streams keyed by seed.  The reference's single chain is unseedable, so
+ * results agree in distribution only. */
+int pl_generate_mcmc_samples(pl_sv_t sv, int kernel, int64_t num_burnin,
+                             int64_t nshots, uint64_t seed, int nchains,
+                             uint64_t *bits);
+
 /* ---- adjoint Jacobian ------------------------------------------------------ */
 
 /* AdjointJacobian::adjointJacobian(jac, JacobianData{ops, obs, tp})
@@ -192,6 +218,15 @@ int pl_sv_norm2(pl_sv_t sv, double *out);
 /* ||a - b||_2 accumulated in FP64; a and b must have the same qubit count and
  * may differ in precision (the error of a complex64 run against complex128). */
 int pl_sv_distance(pl_sv_t a, pl_sv_t b, double *out);
+
+/* VectorJacobianProduct::operator()(jac, JacobianData{ops, tp}, dy)
+ * (VectorJacobianProduct.hpp:77-155): w_t = <dy| d psi / d theta_t> for the
+ * trainable parameters tp (sorted), starting from the final state held by sv
+ * (not modified).  dy: 2^n interleaved complex; vjp: ntp interleaved complex. */
+int pl_vjp(pl_sv_t sv, int nops, const int *ids, const int *nwires,
+           const int *wires, const uint8_t *inverse, const int *nparams,
+           const double *params, const double *dy, int ntp, const int *tp,
+           double *vjp);
 
 /* ---- batched layered circuits ------------------------------------------------ */
 

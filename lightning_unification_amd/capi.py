@@ -30,7 +30,8 @@ EXPORTS = [
     "pl_batch_run", "pl_batch_info", "pl_batch_destroy", "pl_sv_apply_ops_fused", "pl_fuse_tape",
     "pl_sv_norm2", "pl_sv_distance", "pl_mlp_create", "pl_mlp_destroy", "pl_mlp_num_params",
     "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
-    "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train",
+    "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train", "pl_vjp", "pl_expval_sparse",
+    "pl_var_sparse", "pl_generate_mcmc_samples",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -62,6 +63,12 @@ def load():
                                         ctypes.c_void_p, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]
         lib.pl_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        lib.pl_expval_sparse.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        lib.pl_var_sparse.argtypes = lib.pl_expval_sparse.argtypes
+        lib.pl_generate_mcmc_samples.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                                                 ctypes.c_int64, ctypes.c_uint64, ctypes.c_int,
+                                                 ctypes.c_void_p]
         lib.pl_mlp_create.argtypes = [ctypes.c_int] * 5 + [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]
         lib.pl_mlp_load_data.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
         lib.pl_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float,
@@ -196,6 +203,44 @@ class DeviceStateVector:
         out = ctypes.c_double()
         _check(load().pl_sv_distance(self.h, other.h, ctypes.byref(out)))
         return out.value
+
+    def vjp(self, tape, dy, tp):
+        """<dy| d psi / d theta_t> for the trainable parameters tp (pl_vjp)."""
+        ids, nw, wires, inv, npar, par = tape.encode()
+        dyv = np.ascontiguousarray(np.asarray(dy, np.complex128).reshape(-1))
+        if dyv.size != self.size:
+            raise ValueError("dy must have 2^n entries")
+        out = np.zeros(max(1, len(tp)), np.complex128)
+        _check(load().pl_vjp(self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+                             _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)),
+                             _p(_f64(par)), _p(dyv.view(np.float64)), len(tp), _p(_i32(tp)),
+                             _p(out.view(np.float64))))
+        return out[:len(tp)]
+
+    def _sparse(self, fn, row_map, entries, values):
+        rm = np.ascontiguousarray(np.asarray(row_map, np.int64))
+        en = np.ascontiguousarray(np.asarray(entries, np.int64))
+        va = np.ascontiguousarray(np.asarray(values, np.complex128))
+        if en.size != va.size:
+            raise ValueError("entries and values differ in length")
+        out = ctypes.c_double()
+        _check(fn(self.h, _p(rm), rm.size, _p(en if en.size else np.zeros(1, np.int64)),
+                  _p((va if va.size else np.zeros(1, np.complex128)).view(np.float64)), va.size,
+                  ctypes.byref(out)))
+        return out.value
+
+    def expval_sparse(self, row_map, entries, values):
+        return self._sparse(load().pl_expval_sparse, row_map, entries, values)
+
+    def var_sparse(self, row_map, entries, values):
+        return self._sparse(load().pl_var_sparse, row_map, entries, values)
+
+    def generate_mcmc_samples(self, num_shots, kernel=0, num_burnin=100, seed=0, nchains=0):
+        """Metropolis-Hastings samples (pl_generate_mcmc_samples): (shots, n) bits."""
+        out = np.zeros((int(num_shots), self.num_qubits), np.uint64)
+        _check(load().pl_generate_mcmc_samples(self.h, int(kernel), int(num_burnin), int(num_shots),
+                                               int(seed), int(nchains), _p(out)))
+        return out
 
     def apply_matrix(self, matrix, wires, inverse=False):
         m, w = _cmat(matrix), _i32(wires)

@@ -187,10 +187,10 @@ template <class T> struct BatchPlan final : BatchPlanBase {
 
     int lds_limit = 65536; // bytes of LDS a workgroup may use (device attribute)
 
-    // widest tile whose adjoint stage (two tiles + the reduction's 64 B) fits
+    // widest tile whose adjoint stage (two tiles + its 2 KiB of reduction slots) fits
     int max_tile() const {
         int w = 4;
-        while (w < 13 && (size_t{2} << (w + 1)) * sizeof(V) + 256 <= static_cast<size_t>(lds_limit)) {
+        while (w < kBatchMaxTile && (size_t{2} << (w + 1)) * sizeof(V) + 2048 <= static_cast<size_t>(lds_limit)) {
             w++;
         }
         return w;

@@ -154,4 +154,49 @@ template <class StateVectorT> class AdjointJacobian {
     }
 };
 
+/* VectorJacobianProduct (VectorJacobianProduct.hpp:42-157): w = dy^dagger J
+ * for the state-vector Jacobian J_ij = d psi_i / d theta_j. */
+template <class StateVectorT> class VectorJacobianProduct {
+  public:
+    using PrecisionT = typename StateVectorT::PrecisionT;
+    using ComplexT = std::complex<PrecisionT>;
+
+    void operator()(std::span<ComplexT> jac, const JacobianData<StateVectorT> &jd,
+                    std::span<const ComplexT> dy, bool apply_operations = false) {
+        const auto &tp = jd.getTrainableParams();
+        if (dy.size() != jd.getStateVector().getLength()) {
+            throw LightningException("Assertion failed: dy.size() == jd.getSizeStateVec()");
+        }
+        if (tp.empty()) {
+            return;
+        }
+        if (jac.size() != tp.size()) {
+            throw LightningException("The size of preallocated jacobian must be same as the number of "
+                                     "trainable parameters.");
+        }
+        const auto e = jd.getOperations().encode();
+        const std::vector<int> tpi(tp.begin(), tp.end());
+        std::vector<double> d(2 * dy.size()), out(2 * tp.size());
+        for (size_t i = 0; i < dy.size(); i++) {
+            d[2 * i] = dy[i].real();
+            d[2 * i + 1] = dy[i].imag();
+        }
+        const StateVectorT *start = &jd.getStateVector();
+        std::unique_ptr<StateVectorT> applied;
+        if (apply_operations) {
+            applied = std::make_unique<StateVectorT>(jd.getStateVector());
+            check(pl_sv_apply_ops(applied->handle(), static_cast<int>(e.ids.size()), e.ids.data(),
+                                  e.nwires.data(), e.wires.data(), e.inverse.data(), e.nparams.data(),
+                                  e.params.data()));
+            start = applied.get();
+        }
+        check(pl_vjp(start->handle(), static_cast<int>(e.ids.size()), e.ids.data(), e.nwires.data(),
+                     e.wires.data(), e.inverse.data(), e.nparams.data(), e.params.data(), d.data(),
+                     static_cast<int>(tpi.size()), tpi.data(), out.data()));
+        for (size_t t = 0; t < jac.size(); t++) {
+            jac[t] = ComplexT(static_cast<PrecisionT>(out[2 * t]), static_cast<PrecisionT>(out[2 * t + 1]));
+        }
+    }
+};
+
 } // namespace Pennylane::LightningAMD

@@ -74,6 +74,17 @@ template <class StateVectorT> class Measurements final {
         check(pl_expval_matrix(sv_.handle(), m.data(), w.data(), static_cast<int>(w.size()), &out));
         return static_cast<PrecisionT>(out);
     }
+    /* expval / var of a CSR sparse Hamiltonian: MeasurementsLQubit.hpp:183-214, 437-471 */
+    template <class index_type>
+    PrecisionT expval(const index_type *row_map_ptr, const index_type row_map_size,
+                      const index_type *entries_ptr, const ComplexT *values_ptr, const index_type numNNZ) {
+        return sparse(pl_expval_sparse, row_map_ptr, row_map_size, entries_ptr, values_ptr, numNNZ);
+    }
+    template <class index_type>
+    PrecisionT var(const index_type *row_map_ptr, const index_type row_map_size,
+                   const index_type *entries_ptr, const ComplexT *values_ptr, const index_type numNNZ) {
+        return sparse(pl_var_sparse, row_map_ptr, row_map_size, entries_ptr, values_ptr, numNNZ);
+    }
     /* list form: MeasurementsLQubit.hpp:222-246 */
     template <typename op_type>
     std::vector<PrecisionT> expval(const std::vector<op_type> &operations_list,
@@ -95,11 +106,8 @@ template <class StateVectorT> class Measurements final {
     std::vector<size_t> generate_samples(size_t num_samples) {
         const size_t n = sv_.getNumQubits();
         std::vector<uint64_t> bits(num_samples * n);
-        const uint64_t seed = seeded_ ? seed_ : (uint64_t(std::random_device{}()) << 32) ^ std::random_device{}();
-        if (seeded_) {
-            seed_ += 0x9E3779B97F4A7C15ull * (num_samples + 1);
-        }
-        check(pl_generate_samples(sv_.handle(), seed, static_cast<int64_t>(num_samples), bits.data()));
+        check(pl_generate_samples(sv_.handle(), next_seed(num_samples), static_cast<int64_t>(num_samples),
+                                  bits.data()));
         return {bits.begin(), bits.end()};
     }
     /* basis indices for caller-supplied uniforms (inverse CDF, FP64 prefix) */
@@ -108,12 +116,49 @@ template <class StateVectorT> class Measurements final {
         check(pl_sample(sv_.handle(), uniforms.data(), static_cast<int64_t>(uniforms.size()), idx.data()));
         return idx;
     }
+    /* generate_samples_metropolis(kernelname, num_burnin, num_samples):
+     * MeasurementsLQubit.hpp:388-434, with the reference's kernel_factory
+     * mapping of names ("Local" -> proposals uniform over the non-zero
+     * amplitudes, otherwise single-bit flips; TransitionKernels.hpp). */
+    std::vector<size_t> generate_samples_metropolis(const std::string &kernelname, size_t num_burnin,
+                                                    size_t num_samples) {
+        const size_t n = sv_.getNumQubits();
+        std::vector<uint64_t> bits(num_samples * n);
+        const int kind = kernelname == "Local" ? 0 : 1;
+        check(pl_generate_mcmc_samples(sv_.handle(), kind, static_cast<int64_t>(num_burnin),
+                                       static_cast<int64_t>(num_samples), next_seed(num_samples), 0,
+                                       bits.data()));
+        return {bits.begin(), bits.end()};
+    }
     void set_seed(uint64_t seed) {
         seed_ = seed;
         seeded_ = true;
     }
 
   private:
+    uint64_t next_seed(size_t num_samples) {
+        const uint64_t seed = seeded_ ? seed_ : (uint64_t(std::random_device{}()) << 32) ^ std::random_device{}();
+        if (seeded_) {
+            seed_ += 0x9E3779B97F4A7C15ull * (num_samples + 1);
+        }
+        return seed;
+    }
+    template <class F, class index_type>
+    PrecisionT sparse(F fn, const index_type *row_map_ptr, const index_type row_map_size,
+                      const index_type *entries_ptr, const ComplexT *values_ptr, const index_type numNNZ) {
+        std::vector<int64_t> rm(row_map_ptr, row_map_ptr + row_map_size), en(entries_ptr, entries_ptr + numNNZ);
+        std::vector<double> va(2 * static_cast<size_t>(numNNZ) + 2);
+        for (index_type j = 0; j < numNNZ; j++) {
+            va[2 * j] = values_ptr[j].real();
+            va[2 * j + 1] = values_ptr[j].imag();
+        }
+        en.push_back(0);
+        double out = 0;
+        check(fn(sv_.handle(), rm.data(), static_cast<int64_t>(rm.size()), en.data(), va.data(),
+                 static_cast<int64_t>(numNNZ), &out));
+        return static_cast<PrecisionT>(out);
+    }
+
     const StateVectorT &sv_;
     uint64_t seed_ = 0;
     bool seeded_ = false;

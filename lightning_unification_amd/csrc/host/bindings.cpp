@@ -75,6 +75,7 @@ template <class P> void registerPrecision(py::module_ &m, py::module_ &obs_m, py
     using ComplexT = std::complex<P>;
     using np_arr_c = py::array_t<ComplexT, py::array::c_style | py::array::forcecast>;
     using np_arr_r = py::array_t<P, py::array::c_style | py::array::forcecast>;
+    using np_arr_idx = py::array_t<size_t, py::array::c_style | py::array::forcecast>;
     const std::string bits = std::to_string(sizeof(ComplexT) * 8);
 
     /* ---- StateVectorC* (Bindings.hpp:569-614, BindingsBase.hpp:39-83) ---- */
@@ -193,6 +194,24 @@ template <class P> void registerPrecision(py::module_ &m, py::module_ &obs_m, py
             std::copy(result.begin(), result.end(), out.mutable_data());
             return out;
         })
+        .def("expval", [](PyMeas &M, const np_arr_idx &row_map, const np_arr_idx &entries, const np_arr_c &values) {
+            return M.m.expval(row_map.data(), static_cast<size_t>(row_map.size()), entries.data(), values.data(),
+                              static_cast<size_t>(values.size()));
+        }, "Expected value of a sparse Hamiltonian.")
+        .def("var", [](PyMeas &M, const np_arr_idx &row_map, const np_arr_idx &entries, const np_arr_c &values) {
+            return M.m.var(row_map.data(), static_cast<size_t>(row_map.size()), entries.data(), values.data(),
+                           static_cast<size_t>(values.size()));
+        }, "Variance of a sparse Hamiltonian.")
+        .def("generate_mcmc_samples", [](PyMeas &M, size_t num_wires, const std::string &kernelname,
+                                         size_t num_burnin, size_t num_shots) {
+            if (num_wires != M.owner->sv->getNumQubits()) {
+                throw std::invalid_argument("num_wires must equal the number of qubits");
+            }
+            auto result = M.m.generate_samples_metropolis(kernelname, num_burnin, num_shots);
+            py::array_t<size_t> out({num_shots, num_wires});
+            std::copy(result.begin(), result.end(), out.mutable_data());
+            return out;
+        })
         .def("sample_indices", [](PyMeas &M, const py::array_t<double, py::array::c_style | py::array::forcecast> &u) {
             std::vector<double> uv(u.data(), u.data() + u.size());
             return py::array_t<int64_t>(py::cast(M.m.sample_indices(uv)));
@@ -236,6 +255,18 @@ template <class P> void registerPrecision(py::module_ &m, py::module_ &obs_m, py
                  return py::array_t<P>(py::cast(jac));
              },
              "Adjoint Jacobian method.");
+    /* VectorJacobianProductC* (LQubitBindings.hpp:185-246) */
+    py::class_<VectorJacobianProduct<SV>>(alg_m, ("VectorJacobianProductC" + bits).c_str(), py::module_local())
+        .def(py::init<>())
+        .def("__call__",
+             [](VectorJacobianProduct<SV> &vjp, const PySV &s, const Ops &ops, const np_arr_c &dy,
+                const std::vector<size_t> &tp) {
+                 std::vector<ComplexT> out(tp.size(), ComplexT{});
+                 const JacobianData<SV> jd{ops.getTotalNumParams(), *s.sv, {}, ops, tp};
+                 vjp(std::span{out}, jd, std::span<const ComplexT>{dy.data(), static_cast<size_t>(dy.size())});
+                 return py::array_t<ComplexT>(py::cast(out));
+             },
+             "Vector Jacobian Product method.");
 }
 
 PYBIND11_MODULE(lightning_qubit_ops, m) {

@@ -26,6 +26,7 @@ namespace plamd {
 
 constexpr int kBatchMaxQubits = 30;
 constexpr int kBatchMaxPattern = 4;
+constexpr int kBatchMaxTile = 13; // widest LDS tile (bits)
 
 struct BCirc {
     int n;           // qubits
@@ -311,9 +312,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
 
 // Adjoint rotation stage: lambda (buffer lb) and H (buffer hb) tiles in LDS.
 // For each target wire, pattern rotations are undone last-to-first; before
-// undoing rotation a, Im<H|G_a lambda> is accumulated into grad[angle]
+// undoing rotation a, Im<H|G_a lambda> is accumulated for angle a
 // (scale -1/2 and the -2 of the reference's update cancel: dE/dtheta =
-// Im<H|G lambda>, AdjointJacobianLQubit.hpp:70-75).
+// Im<H|G lambda>, AdjointJacobianLQubit.hpp:70-75).  Each wave reduces its
+// share with shuffles into a per-wave LDS slot; after the last wire one pass
+// adds the four waves' slots and issues one FP64 atomic per angle.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ arena,
                                                           const BCirc *__restrict__ circs,
@@ -322,6 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
                                                           int4 pat, const double *__restrict__ params,
                                                           double *__restrict__ grads) {
     extern __shared__ unsigned char smem_raw[];
+    __shared__ double wpart[kBlock / 64][kBatchMaxTile][kBatchMaxPattern];
     uint64_t q;
     const BCirc &c = circs[find_circ(L, blockIdx.x, q)];
     int bits[16];
@@ -341,6 +345,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
     }
     __syncthreads();
     const int axes[4] = {pat.x, pat.y, pat.z, pat.w};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int slot = 0; // target wires done so far: slot j is bit alo + j
     for (int k = 0; k < Tn; k++) {
         const int b = bits[k];
         if (b < alo || b >= ahi) {
@@ -379,12 +385,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
             th_[i1] = h1;
         }
         for (int a = 0; a < npat; a++) {
-            const double2 s = block_sum2(double2{acc[a], 0});
-            if (threadIdx.x == 0) {
-                atomicAdd(grads + pidx + a, s.x);
+            const double s = wave_sum(acc[a]);
+            if (lane == 0) {
+                wpart[wave][slot][a] = s;
             }
         }
+        slot++;
         __syncthreads();
+    }
+    for (int t = threadIdx.x; t < slot * npat; t += kBlock) {
+        const int sl = t / npat, a = t - sl * npat;
+        double s = 0;
+        for (int v = 0; v < kBlock / 64; v++) {
+            s += wpart[v][sl][a];
+        }
+        const int w = c.n - 1 - (alo + sl);
+        atomicAdd(grads + c.par + (static_cast<uint64_t>(layer) * c.n + w) * npat + a, s);
     }
     for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
         const uint64_t g = tile_index(base, e, bits, Tn);

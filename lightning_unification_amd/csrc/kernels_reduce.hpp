@@ -271,4 +271,34 @@ __global__ __launch_bounds__(kBlock) void k_marginal_small(const v2<T> *__restri
     out[bin] = s;
 }
 
+// Sparse (CSR) Hamiltonian moments in one pass, y = H psi row by row
+// (SparseLinAlg.hpp apply_Sparse_Matrix):
+//   partial.x = sum_i Re(conj(psi_i) y_i)   (the expectation value)
+//   partial.y = sum_i |y_i|^2               (<psi|H^dag H|psi>)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_sparse_moments(const v2<T> *__restrict__ sv, uint64_t len,
+                                                           const int64_t *__restrict__ row_map,
+                                                           const int64_t *__restrict__ cols,
+                                                           const dcx<T> *__restrict__ vals,
+                                                           double2 *__restrict__ partial) {
+    double2 acc{0, 0};
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len;
+         i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        double yr = 0, yi = 0;
+        for (int64_t j = row_map[i]; j < row_map[i + 1]; j++) {
+            const dcx<T> a = vals[j];
+            const v2<T> x = sv[cols[j]];
+            yr += double(a.r) * double(x.x) - double(a.i) * double(x.y);
+            yi += double(a.r) * double(x.y) + double(a.i) * double(x.x);
+        }
+        const v2<T> p = sv[i];
+        acc.x += double(p.x) * yr + double(p.y) * yi;
+        acc.y += yr * yr + yi * yi;
+    }
+    const double2 r = block_sum2(acc);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = r;
+    }
+}
+
 } // namespace plamd

@@ -15,6 +15,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/pl_amd.h"
@@ -23,6 +24,7 @@
 #include "kernels_reduce.hpp"
 #include "kernels_sample.hpp"
 #include "kernels_mfma.hpp"
+#include "kernels_mcmc.hpp"
 #include "fuse.hpp"
 #include "lower.hpp"
 
@@ -269,6 +271,7 @@ void *stage(pl_sv *h, const void *src, size_t bytes) {
 
 template <class T> struct Engine {
     using V = v2<T>;
+    using Scalar = T;
     pl_sv *h;
     explicit Engine(pl_sv *hh) : h(hh) {}
     V *psi() const { return static_cast<V *>(h->d); }
@@ -917,6 +920,71 @@ template <class T> struct Engine {
         HIP_OK(hipStreamSynchronize(h->stream));
     }
 
+    // Metropolis-Hastings samples (kernels_mcmc.hpp); bits as generate_samples
+    void mcmc(int kind, int64_t burnin, int64_t m, uint64_t seed, int nchains, uint64_t *host_bits) {
+        if (kind != 0 && kind != 1) {
+            throw InvalidArg("transition kernel must be 0 (non-zero uniform) or 1 (local flip)");
+        }
+        if (burnin < 0 || m < 1) {
+            throw InvalidArg("need num_burnin >= 0 and num_shots >= 1");
+        }
+        struct Free {
+            std::vector<void *> p;
+            ~Free() {
+                for (void *q : p) {
+                    (void)hipFree(q);
+                }
+            }
+        } fr;
+        auto alloc = [&](size_t bytes) {
+            void *q = nullptr;
+            HIP_OK(hipMalloc(&q, std::max<size_t>(bytes, 8)));
+            fr.p.push_back(q);
+            return q;
+        };
+        uint64_t *nz = nullptr;
+        uint64_t nnz = len();
+        if (kind == 0) {
+            const uint64_t nb = (len() + kNzChunk - 1) / kNzChunk;
+            auto *counts = static_cast<double *>(alloc(nb * sizeof(double)));
+            auto *offs = static_cast<double *>(alloc((nb + 1) * sizeof(double)));
+            const T eps = std::numeric_limits<T>::epsilon();
+            hipLaunchKernelGGL((k_nz_count<T>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, h->stream,
+                               psi(), len(), eps, counts);
+            HIP_OK(hipGetLastError());
+            hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, h->stream, counts, nb, offs);
+            HIP_OK(hipGetLastError());
+            double total = 0;
+            HIP_OK(hipMemcpyAsync(&total, offs + nb, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+            HIP_OK(hipStreamSynchronize(h->stream));
+            nnz = static_cast<uint64_t>(total);
+            if (nnz == 0) {
+                throw InvalidArg("the state has no amplitude above the precision's epsilon");
+            }
+            if (nnz < len()) {
+                nz = static_cast<uint64_t *>(alloc(nnz * sizeof(uint64_t)));
+                hipLaunchKernelGGL((k_nz_write<T>), dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, h->stream,
+                                   psi(), len(), eps, offs, nz);
+                HIP_OK(hipGetLastError());
+            }
+        }
+        const uint64_t chains = nchains > 0 ? std::min<uint64_t>(nchains, m)
+                                            : std::min<uint64_t>(m, uint64_t{16384});
+        const uint64_t per = (m + chains - 1) / chains;
+        auto *didx = static_cast<int64_t *>(alloc(m * sizeof(int64_t)));
+        hipLaunchKernelGGL((k_mcmc_chains<T>), dim3(static_cast<unsigned>((chains + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, h->stream, psi(), len(), n(), kind, nz, nnz,
+                           static_cast<uint64_t>(burnin), per, static_cast<uint64_t>(m), seed, didx);
+        HIP_OK(hipGetLastError());
+        const uint64_t tot = uint64_t(m) * n();
+        auto *dbits = static_cast<uint64_t *>(alloc(tot * sizeof(uint64_t)));
+        hipLaunchKernelGGL(k_sample_bits, dim3((tot + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream, didx, m,
+                           n(), dbits);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(host_bits, dbits, tot * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+
     // Measurements::expval(Observable) and var(Observable)
     double expval(const ObsNode &o, bool var) {
         PauliWord pw;
@@ -963,6 +1031,18 @@ template <class F> int guarded(F &&f) {
     } catch (const std::exception &e) {
         g_err = e.what();
         return PL_ERR_INVALID;
+    }
+}
+
+// Engine of the handle's precision, exceptions propagate
+template <class F> void dispatch_or_throw(pl_sv_t h, F &&f) {
+    HIP_OK(hipSetDevice(0));
+    if (h->dtype == PL_C128) {
+        Engine<double> e(h);
+        f(e);
+    } else {
+        Engine<float> e(h);
+        f(e);
     }
 }
 
@@ -1227,6 +1307,84 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
 // ===========================================================================
 // extern "C"
 // ===========================================================================
+// VectorJacobianProduct::operator() (VectorJacobianProduct.hpp:77-155):
+// lambda = psi, mu = dy; for operations in reverse (state preparations
+// skipped, stopping once every trainable parameter is done): for a trainable
+// one, w_t = i s <G mu | lambda> with s the generator scale times -1 for an
+// inverted operation; then lambda <- U^dag lambda and mu <- U^dag mu.
+template <class T>
+void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const int *wires,
+               const uint8_t *inverse, const int *nparams, const double *params, const double *dy,
+               int ntp, const int *tp, double *out) {
+    using V = v2<T>;
+    pl_sv *h = E.h;
+    std::vector<int> woff(nops + 1, 0), poff(nops + 1, 0);
+    size_t num_par_ops = 0;
+    for (int i = 0; i < nops; i++) {
+        woff[i + 1] = woff[i] + nwires[i];
+        poff[i + 1] = poff[i] + nparams[i];
+        num_par_ops += nparams[i] > 0 ? 1 : 0;
+        if (ids[i] != -1) {
+            gate_name(ids[i]);
+        }
+    }
+    for (int t = 1; t < ntp; t++) {
+        if (tp[t] <= tp[t - 1]) {
+            throw InvalidArg("trainable parameter indices must be sorted ascending");
+        }
+    }
+    if (ntp == 0) {
+        return;
+    }
+    ScratchScope sc(h);
+    V *lam = static_cast<V *>(sc.get());
+    V *mu = static_cast<V *>(sc.get());
+    V *mud = static_cast<V *>(sc.get());
+    E.copy(lam, E.psi());
+    {
+        std::vector<V> host(E.len());
+        for (uint64_t i = 0; i < E.len(); i++) {
+            host[i].x = static_cast<T>(dy[2 * i]);
+            host[i].y = static_cast<T>(dy[2 * i + 1]);
+        }
+        HIP_OK(hipMemcpyAsync(mu, host.data(), E.len() * sizeof(V), hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+    long long cur_param = static_cast<long long>(num_par_ops) - 1;
+    int tp_pos = ntp - 1;
+    for (int i = nops - 1; i >= 0; i--) {
+        if (nparams[i] > 1) {
+            throw InvalidArg("The operation is not supported using the adjoint differentiation method");
+        }
+        if (ids[i] == -1) {
+            continue;
+        }
+        if (tp_pos < 0) {
+            break;
+        }
+        const int *w = wires + woff[i];
+        if (nparams[i] > 0) {
+            if (cur_param == tp[tp_pos]) {
+                E.copy(mud, mu);
+                T s = 0;
+                E.apply(lower_generator<T>(E.n(), ids[i], w, nwires[i], false, &s), mud);
+                const double sc_f = static_cast<double>(s) * (inverse[i] ? -1.0 : 1.0);
+                const std::complex<double> z = E.dot(mud, lam);
+                out[2 * tp_pos] = -sc_f * z.imag();
+                out[2 * tp_pos + 1] = sc_f * z.real();
+                tp_pos--;
+            }
+            cur_param--;
+        }
+        const auto pt = E.params_t(params + poff[i], nparams[i]);
+        const GroupOp<T> ud = lower_gate<T>(E.n(), ids[i], w, nwires[i], inverse[i] == 0, pt.data(),
+                                            nparams[i]);
+        E.apply(ud, lam);
+        E.apply(ud, mu);
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+}
+
 extern "C" {
 
 const char *pl_last_error(void) { return g_err.c_str(); }
@@ -1584,6 +1742,106 @@ int pl_adjoint_jacobian(pl_sv_t h, int nops, const int *ids, const int *nwires, 
     return dispatch(h, [&](auto &E) {
         adjoint_sweep(E, nops, ids, nwires, wires, inverse, nparams, params, prog, coeffs, mats,
                       ntp, tp, jac);
+    });
+}
+
+int pl_vjp(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
+           const uint8_t *inverse, const int *nparams, const double *params, const double *dy,
+           int ntp, const int *tp, double *vjp) {
+    return dispatch(h, [&](auto &E) {
+        if (!dy || (ntp > 0 && (!tp || !vjp)) || nops < 0) {
+            throw InvalidArg("null argument");
+        }
+        vjp_sweep(E, nops, ids, nwires, wires, inverse, nparams, params, dy, ntp, tp, vjp);
+    });
+}
+
+// Measurements::expval / var of a CSR matrix (MeasurementsLQubit.hpp:183-214,
+// 437-471; SparseLinAlg.hpp apply_Sparse_Matrix): both from one fused pass.
+static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_size,
+                           const int64_t *entries, const double *values, int64_t nnz, double *mean,
+                           double *mean_square) {
+    if (!row_map || (nnz > 0 && (!entries || !values)) || nnz < 0) {
+        throw InvalidArg("null argument");
+    }
+    if (static_cast<uint64_t>(row_map_size) != h->len + 1) {
+        throw InvalidArg("Statevector and Hamiltonian have incompatible sizes.");
+    }
+    if (row_map[0] != 0 || row_map[row_map_size - 1] != nnz) {
+        throw InvalidArg("row_map must start at 0 and end at the number of non-zeros");
+    }
+    for (int64_t i = 0; i + 1 < row_map_size; i++) {
+        if (row_map[i + 1] < row_map[i]) {
+            throw InvalidArg("row_map must be non-decreasing");
+        }
+    }
+    for (int64_t j = 0; j < nnz; j++) {
+        if (entries[j] < 0 || static_cast<uint64_t>(entries[j]) >= h->len) {
+            throw InvalidArg("column index out of range");
+        }
+    }
+    dispatch_or_throw(h, [&](auto &E) {
+        using T = typename std::remove_reference_t<decltype(E)>::Scalar;
+        std::vector<dcx<T>> vals(std::max<int64_t>(nnz, 1));
+        for (int64_t j = 0; j < nnz; j++) {
+            vals[j] = {static_cast<T>(values[2 * j]), static_cast<T>(values[2 * j + 1])};
+        }
+        unsigned char *dev = nullptr;
+        const size_t rb = row_map_size * 8, eb = std::max<int64_t>(nnz, 1) * 8, vb = vals.size() * sizeof(dcx<T>);
+        HIP_OK(hipMalloc(&dev, rb + eb + vb));
+        struct Free {
+            unsigned char *p;
+            ~Free() { (void)hipFree(p); }
+        } fr{dev};
+        HIP_OK(hipMemcpyAsync(dev, row_map, rb, hipMemcpyHostToDevice, h->stream));
+        if (nnz > 0) {
+            HIP_OK(hipMemcpyAsync(dev + rb, entries, nnz * 8, hipMemcpyHostToDevice, h->stream));
+        }
+        HIP_OK(hipMemcpyAsync(dev + rb + eb, vals.data(), vb, hipMemcpyHostToDevice, h->stream));
+        const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 4), kReduceBlocks);
+        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + 1) * sizeof(double2)));
+        hipLaunchKernelGGL((k_sparse_moments<T>), dim3(grid), dim3(kBlock), 0, h->stream, E.psi(), E.len(),
+                           reinterpret_cast<const int64_t *>(dev), reinterpret_cast<const int64_t *>(dev + rb),
+                           reinterpret_cast<const dcx<T> *>(dev + rb + eb), part);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, (int)grid,
+                           part + kReduceBlocks);
+        HIP_OK(hipGetLastError());
+        double2 r;
+        HIP_OK(hipMemcpyAsync(&r, part + kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        *mean = r.x;
+        *mean_square = r.y;
+    });
+}
+
+int pl_expval_sparse(pl_sv_t h, const int64_t *row_map, int64_t row_map_size, const int64_t *entries,
+                     const double *values, int64_t nnz, double *out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        double m = 0, ms = 0;
+        sparse_moments(h, row_map, row_map_size, entries, values, nnz, &m, &ms);
+        *out = m;
+    });
+}
+
+int pl_var_sparse(pl_sv_t h, const int64_t *row_map, int64_t row_map_size, const int64_t *entries,
+                  const double *values, int64_t nnz, double *out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        double m = 0, ms = 0;
+        sparse_moments(h, row_map, row_map_size, entries, values, nnz, &m, &ms);
+        *out = ms - m * m;
+    });
+}
+
+int pl_generate_mcmc_samples(pl_sv_t h, int kernel, int64_t num_burnin, int64_t nshots, uint64_t seed,
+                             int nchains, uint64_t *bits) {
+    return dispatch(h, [&](auto &E) {
+        if (!bits) {
+            throw InvalidArg("null output");
+        }
+        E.mcmc(kernel, num_burnin, nshots, seed, nchains, bits);
     });
 }
 

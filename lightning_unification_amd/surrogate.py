@@ -153,3 +153,30 @@ def optimize(problem, theta0, target, max_calls=400, eta=0.05, surrogate=None, c
             if tried:
                 break
     return np.array(hist), problem.calls - start_calls, theta
+
+
+def r2(pred, true):
+    """Coefficient of determination over all entries."""
+    pred, true = np.asarray(pred), np.asarray(true)
+    return float(1.0 - np.mean((pred - true) ** 2) / np.var(true))
+
+
+def screen_gain(problem, surrogate, rounds=16, candidates=256, seed=2024):
+    """Screening value of the network: each round draws `candidates` uniform
+    parameter vectors, the network picks the one with the lowest predicted
+    energy, and the exact simulation scores every candidate (one batched
+    launch).  Returns mean exact energy of the network's pick, of a random
+    pick (the candidates' mean), of the exact best, and the fraction of
+    rounds in which the pick beats the candidates' median."""
+    pick_e, mean_e, best_e, wins = [], [], [], 0
+    for r in range(rounds):
+        cand = sample_thetas(candidates, problem.nparams, seed + r)
+        e_hat, _ = surrogate.predict(cand)
+        e, _ = problem.exact(cand)
+        k = int(np.argmin(e_hat))
+        pick_e.append(e[k])
+        mean_e.append(e.mean())
+        best_e.append(e.min())
+        wins += e[k] < np.median(e)
+    return {"pick": float(np.mean(pick_e)), "random": float(np.mean(mean_e)),
+            "best": float(np.mean(best_e)), "win_rate": wins / rounds}

@@ -25,6 +25,11 @@
  *                     unseedable) cannot be compared draw by draw.
  *   adjoint Jacobian  AdjointJacobianLQubit.hpp:212-336
  *                     -> pinned against oracle/_ref and parameter shift.
+ *   VJP               VectorJacobianProduct.hpp:77-155
+ *                     -> pinned against oracle/_ref, central differences of
+ *                     the forward pass, and 2 Re VJP(O psi) = adjoint row.
+ *   sparse H          MeasurementsLQubit.hpp:183-214,437-471 + SparseLinAlg.hpp
+ *                     -> pinned against oracle/_ref and dense numpy algebra.
  * Wire w is bit (n-1-w) of the basis index (GateImplementationsLM.hpp:209).
  */
 #include <algorithm>
@@ -1039,6 +1044,86 @@ int adjoint(const P *sv, int n, int nops, const int *ids, const int *nw, const i
 
 } // namespace
 
+/* VectorJacobianProduct::operator() (VectorJacobianProduct.hpp:77-155):
+ * lambda = psi, mu = dy; reverse sweep; w_t = i s <G mu|lambda>. */
+template <class P>
+int vjp(const P *sv, int n, int nops, const int *ids, const int *nw, const int *w,
+        const uint8_t *iv, const int *np, const double *pr, const P *dy, int ntp, const int *tp,
+        P *out) {
+    return guard([&] {
+        Tape<P> ops(nops, ids, nw, w, iv, np, pr);
+        if (ntp == 0) {
+            return;
+        }
+        const size_t len = size_t{1} << n;
+        size_t num_par_ops = 0;
+        for (auto &p : ops.params) {
+            num_par_ops += p.empty() ? 0 : 1;
+        }
+        std::vector<C<P>> lambda(cp(sv), cp(sv) + len), mu(cp(dy), cp(dy) + len), mud(len);
+        SV<P> L{lambda.data(), static_cast<size_t>(n)}, M{mu.data(), static_cast<size_t>(n)};
+        size_t cur_param = num_par_ops - 1;
+        int tp_pos = ntp - 1;
+        for (int op = nops - 1; op >= 0; op--) {
+            if (ops.params[op].size() > 1) {
+                fail("The operation is not supported using the adjoint differentiation method");
+            }
+            if (ops.ids[op] == -1) {
+                continue;
+            }
+            if (tp_pos < 0) {
+                break;
+            }
+            if (!ops.params[op].empty()) {
+                if (cur_param == static_cast<size_t>(tp[tp_pos])) {
+                    std::copy(mu.begin(), mu.end(), mud.begin());
+                    SV<P> D{mud.data(), static_cast<size_t>(n)};
+                    const P scale = apply_generator(D, ops.ids[op], ops.wires[op]) *
+                                    (ops.inv[op] ? P{-1} : P{1});
+                    const C<P> z = C<P>{0, scale} * inner(mud.data(), lambda.data(), len);
+                    out[2 * tp_pos] = z.real();
+                    out[2 * tp_pos + 1] = z.imag();
+                    tp_pos--;
+                }
+                cur_param--;
+            }
+            apply_gate(L, ops.ids[op], ops.wires[op], !ops.inv[op], ops.params[op]);
+            apply_gate(M, ops.ids[op], ops.wires[op], !ops.inv[op], ops.params[op]);
+        }
+    });
+}
+
+/* Measurements::expval / var of a CSR matrix (MeasurementsLQubit.hpp:183-214,
+ * 437-471): y = H psi row by row (SparseLinAlg.hpp apply_Sparse_Matrix), then
+ * Re <psi|y> and <y|y> - Re<y|psi>^2. */
+template <class P>
+int sparse(const P *sv, int n, const int64_t *rm, int64_t rms, const int64_t *en, const P *va,
+           int64_t nnz, double *out, bool var) {
+    return guard([&] {
+        const size_t len = size_t{1} << n;
+        if (static_cast<size_t>(rms) != len + 1) {
+            fail("Statevector and Hamiltonian have incompatible sizes.");
+        }
+        (void)nnz;
+        const C<P> *psi = cp(sv), *v = cp(va);
+        std::vector<C<P>> y(len);
+        for (size_t i = 0; i < len; i++) {
+            C<P> acc = 0;
+            for (int64_t j = rm[i]; j < rm[i + 1]; j++) {
+                acc += v[j] * psi[en[j]];
+            }
+            y[i] = acc;
+        }
+        if (!var) {
+            *out = std::real(inner(psi, y.data(), len));
+        } else {
+            const P ms = std::real(inner(y.data(), y.data(), len));
+            const P sm = std::real(inner(y.data(), psi, len));
+            *out = ms - sm * sm;
+        }
+    });
+}
+
 extern "C" {
 const char *orc_last_error(void) { return g_err.c_str(); }
 int orc_num_threads(void) {
@@ -1111,4 +1196,31 @@ int orc_num_threads(void) {
 
 ORC_DEF(c128, double)
 ORC_DEF(c64, float)
+
+int orc_vjp_c128(const double *sv, int n, int nops, const int *ids, const int *nw, const int *w,
+                 const uint8_t *iv, const int *np, const double *pr, const double *dy, int ntp,
+                 const int *tp, double *out) {
+    return vjp<double>(sv, n, nops, ids, nw, w, iv, np, pr, dy, ntp, tp, out);
+}
+int orc_vjp_c64(const float *sv, int n, int nops, const int *ids, const int *nw, const int *w,
+                const uint8_t *iv, const int *np, const double *pr, const float *dy, int ntp,
+                const int *tp, float *out) {
+    return vjp<float>(sv, n, nops, ids, nw, w, iv, np, pr, dy, ntp, tp, out);
+}
+int orc_expval_sparse_c128(const double *sv, int n, const int64_t *rm, int64_t rms,
+                           const int64_t *en, const double *va, int64_t nnz, double *out) {
+    return sparse<double>(sv, n, rm, rms, en, va, nnz, out, false);
+}
+int orc_expval_sparse_c64(const float *sv, int n, const int64_t *rm, int64_t rms,
+                          const int64_t *en, const float *va, int64_t nnz, double *out) {
+    return sparse<float>(sv, n, rm, rms, en, va, nnz, out, false);
+}
+int orc_var_sparse_c128(const double *sv, int n, const int64_t *rm, int64_t rms,
+                        const int64_t *en, const double *va, int64_t nnz, double *out) {
+    return sparse<double>(sv, n, rm, rms, en, va, nnz, out, true);
+}
+int orc_var_sparse_c64(const float *sv, int n, const int64_t *rm, int64_t rms,
+                       const int64_t *en, const float *va, int64_t nnz, double *out) {
+    return sparse<float>(sv, n, rm, rms, en, va, nnz, out, true);
+}
 }

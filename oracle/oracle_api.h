@@ -74,7 +74,18 @@ int orc_num_threads(void);
                           const uint8_t *inverse, const int *nparams,         \
                           const double *params, const int *prog,              \
                           const double *coeffs, const REAL *mats, int ntp,    \
-                          const int *tp, REAL *jac);
+                          const int *tp, REAL *jac);                          \
+    int orc_vjp_##SUF(const REAL *sv, int n, int nops, const int *ids,        \
+                      const int *nwires, const int *wires,                    \
+                      const uint8_t *inverse, const int *nparams,             \
+                      const double *params, const REAL *dy, int ntp,          \
+                      const int *tp, REAL *vjp);                              \
+    int orc_expval_sparse_##SUF(const REAL *sv, int n, const int64_t *row_map,\
+                                int64_t row_map_size, const int64_t *entries, \
+                                const REAL *values, int64_t nnz, double *out);\
+    int orc_var_sparse_##SUF(const REAL *sv, int n, const int64_t *row_map,   \
+                             int64_t row_map_size, const int64_t *entries,    \
+                             const REAL *values, int64_t nnz, double *out);
 
 ORC_DECL(c128, double)
 ORC_DECL(c64, float)

@@ -153,3 +153,30 @@ class Oracle:
                      _p(npar), _p(par), _p(prog), _p(coeffs), _p(mats), len(tp), _p(tpa),
                      _p(jac)))
         return jac[: len(obs_list) * len(tp)].reshape(len(obs_list), len(tp))
+
+    def vjp(self, sv, tape, dy, tp):
+        """<dy| d psi / d theta_t> (VectorJacobianProduct.hpp:77-155)."""
+        suf, real = self._suf(sv)
+        ids, nw, wires, inv, npar, par = tape.encode()
+        wires = wires if wires.size else np.zeros(1, np.int32)
+        par = par if par.size else np.zeros(1)
+        d = np.ascontiguousarray(np.asarray(dy, sv.dtype))
+        tpa = np.array(tp if len(tp) else [0], np.int32)
+        out = np.zeros(2 * max(1, len(tp)), real)
+        fn = getattr(self.lib, f"orc_vjp_{suf}")
+        self._chk(fn(_p(sv), self.nqubits(sv), len(ids), _p(ids), _p(nw), _p(wires), _p(inv),
+                     _p(npar), _p(par), _p(d), len(tp), _p(tpa), _p(out)))
+        return out[:2 * len(tp)].astype(np.float64).view(np.complex128)
+
+    def sparse(self, sv, row_map, entries, values, var=False):
+        """<psi|H|psi> or Var(H) for a CSR matrix (MeasurementsLQubit.hpp:183-214,437-471)."""
+        suf, _ = self._suf(sv)
+        rm = np.ascontiguousarray(row_map, np.int64)
+        en = np.ascontiguousarray(entries if len(entries) else [0], np.int64)
+        va = np.ascontiguousarray(values if len(values) else [0], sv.dtype)
+        out = ctypes.c_double()
+        fn = getattr(self.lib, f"orc_{'var' if var else 'expval'}_sparse_{suf}")
+        self._chk(fn(_p(sv), self.nqubits(sv), _p(rm), ctypes.c_int64(rm.size), _p(en), _p(va),
+                     ctypes.c_int64(len(values)), ctypes.byref(out)))
+        return out.value
+

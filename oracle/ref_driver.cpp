@@ -23,6 +23,7 @@
 #include "MeasurementsLQubit.hpp"
 #include "ObservablesLQubit.hpp"
 #include "StateVectorLQubitRaw.hpp"
+#include "VectorJacobianProduct.hpp"
 
 #include "oracle_api.h"
 
@@ -253,6 +254,37 @@ int adjoint(const P *sv, int n, int nops, const int *ids, const int *nwires,
         std::copy(j.begin(), j.end(), jac);
     });
 }
+template <class P>
+int vjp(const P *sv, int n, int nops, const int *ids, const int *nwires, const int *wires,
+        const uint8_t *inverse, const int *nparams, const double *params, const P *dy, int ntp,
+        const int *tp, P *out) {
+    return guard([&] {
+        SV<P> s(as_c(const_cast<P *>(sv)), size_t{1} << n);
+        auto ops = make_ops<P>(nops, ids, nwires, wires, inverse, nparams, params);
+        std::vector<size_t> tps(tp, tp + ntp);
+        std::vector<std::complex<P>> j(tps.size());
+        Pennylane::Algorithms::JacobianData<SV<P>> jd{ops.getTotalNumParams(), s.getLength(),
+                                                      s.getData(), {}, ops, tps};
+        LQA::VectorJacobianProduct<SV<P>> v;
+        const auto *d = reinterpret_cast<const std::complex<P> *>(dy);
+        v(std::span{j}, jd, std::span<const std::complex<P>>{d, s.getLength()});
+        for (size_t t = 0; t < j.size(); t++) {
+            out[2 * t] = j[t].real();
+            out[2 * t + 1] = j[t].imag();
+        }
+    });
+}
+
+template <class P>
+int sparse(const P *sv, int n, const int64_t *rm, int64_t rms, const int64_t *en, const P *va,
+           int64_t nnz, double *out, bool var) {
+    return guard([&] {
+        SV<P> s(as_c(const_cast<P *>(sv)), size_t{1} << n);
+        Pennylane::LightningQubit::Measures::Measurements<SV<P>> m(s);
+        const auto *v = reinterpret_cast<const std::complex<P> *>(va);
+        *out = var ? m.var(rm, rms, en, v, nnz) : m.expval(rm, rms, en, v, nnz);
+    });
+}
 } // namespace
 
 extern "C" {
@@ -309,4 +341,31 @@ int orc_num_threads(void) { return 0; }
 
 ORC_DEF(c128, double)
 ORC_DEF(c64, float)
+
+int orc_vjp_c128(const double *sv, int n, int nops, const int *ids, const int *nw, const int *w,
+                 const uint8_t *iv, const int *np, const double *pr, const double *dy, int ntp,
+                 const int *tp, double *out) {
+    return vjp<double>(sv, n, nops, ids, nw, w, iv, np, pr, dy, ntp, tp, out);
+}
+int orc_vjp_c64(const float *sv, int n, int nops, const int *ids, const int *nw, const int *w,
+                const uint8_t *iv, const int *np, const double *pr, const float *dy, int ntp,
+                const int *tp, float *out) {
+    return vjp<float>(sv, n, nops, ids, nw, w, iv, np, pr, dy, ntp, tp, out);
+}
+int orc_expval_sparse_c128(const double *sv, int n, const int64_t *rm, int64_t rms,
+                           const int64_t *en, const double *va, int64_t nnz, double *out) {
+    return sparse<double>(sv, n, rm, rms, en, va, nnz, out, false);
+}
+int orc_expval_sparse_c64(const float *sv, int n, const int64_t *rm, int64_t rms,
+                          const int64_t *en, const float *va, int64_t nnz, double *out) {
+    return sparse<float>(sv, n, rm, rms, en, va, nnz, out, false);
+}
+int orc_var_sparse_c128(const double *sv, int n, const int64_t *rm, int64_t rms,
+                        const int64_t *en, const double *va, int64_t nnz, double *out) {
+    return sparse<double>(sv, n, rm, rms, en, va, nnz, out, true);
+}
+int orc_var_sparse_c64(const float *sv, int n, const int64_t *rm, int64_t rms,
+                       const int64_t *en, const float *va, int64_t nnz, double *out) {
+    return sparse<float>(sv, n, rm, rms, en, va, nnz, out, true);
+}
 }
