@@ -12,8 +12,8 @@
 // include bits 0..3.  LDS holds the tile as [target index j][column c].  Wave
 // w multiplies U into columns 32w .. 32w+31: for each K pair (j, j+1) a lane
 // supplies A = U[row = lane % 32][j + lane / 32] and B = x[j + lane / 32][c],
-// and the products of Ur and Ui with Re x and Im x go to four independent
-// accumulators (so consecutive MFMAs do not wait on one another).
+// and the real and imaginary outputs accumulate in two chains whose MFMAs
+// alternate (so consecutive MFMAs do not wait on one another).
 //
 // FP64 is kept where the precision form needs it: each tile's sum of |psi'|^2
 // is accumulated in double and written to a per-tile partial; a one-block
@@ -58,8 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_fused5_mfma(float2 *__restrict__ sv,
                                                         const Fused5Mat *__restrict__ mat,
                                                         const double *__restrict__ scale,
                                                         double *__restrict__ partial) {
-    __shared__ float2 tile[kF5Tile];
-    __shared__ double wsum[kBlock / 64];
+    __shared__ float2 tile[kF5Tile]; // exactly 32 KiB: five workgroups per CU
     const uint64_t q = xcd_block();
     if (q >= ntiles) {
         return; // uniform per workgroup
@@ -106,16 +105,16 @@ __global__ __launch_bounds__(kBlock) void k_fused5_mfma(float2 *__restrict__ sv,
     const int lane = tid & 63, wave = tid >> 6;
     const int col = wave * 32 + (lane & 31), half = lane >> 5;
     const float s = scale ? static_cast<float>(*scale) : 1.0f;
-    floatx16 rr = {}, ri = {}, ir = {}, ii = {}; // Ur.Re, Ui.Im, Ui.Re, Ur.Im
+    floatx16 re = {}, im = {}; // two chains, alternating
 #pragma unroll
     for (int st = 0; st < 16; st++) {
         const float2 a = mat->a[lane][st];
         const float2 x = tile[(2 * st + half) * kF5Cols + col];
         const float ur = a.x * s, ui = a.y * s;
-        rr = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.x, rr, 0, 0, 0);
-        ri = __builtin_amdgcn_mfma_f32_32x32x2f32(ui, x.y, ri, 0, 0, 0);
-        ir = __builtin_amdgcn_mfma_f32_32x32x2f32(ui, x.x, ir, 0, 0, 0);
-        ii = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.y, ii, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.x, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x2f32(ui, x.x, im, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_32x32x2f32(-ui, x.y, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_32x32x2f32(ur, x.y, im, 0, 0, 0);
     }
     // every lane of this wave has read its operands from columns it alone
     // writes below; the wave's LDS accesses complete in program order
@@ -124,8 +123,8 @@ __global__ __launch_bounds__(kBlock) void k_fused5_mfma(float2 *__restrict__ sv,
     for (int i = 0; i < 16; i++) {
         const int row = 8 * (i >> 2) + 4 * half + (i & 3);
         float2 y;
-        y.x = rr[i] - ri[i];
-        y.y = ir[i] + ii[i];
+        y.x = re[i];
+        y.y = im[i];
         tile[row * kF5Cols + col] = y;
         nrm += static_cast<double>(y.x) * y.x + static_cast<double>(y.y) * y.y;
     }
@@ -144,7 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_fused5_mfma(float2 *__restrict__ sv,
         st_nt<float>(sv + a, tile[l]);
     }
     if (partial) {
+        // the wave sums go into the tile's LDS once every wave has stored
         nrm = wave_sum(nrm);
+        __syncthreads();
+        double *wsum = reinterpret_cast<double *>(tile);
         if (lane == 0) {
             wsum[wave] = nrm;
         }
