@@ -487,16 +487,16 @@ template <class T> struct Engine {
     // ---- precision path: fused blocks -----------------------------------------
     static constexpr bool kMfma = std::is_same_v<T, float>;
     bool mfma_ok(const FusedBlock &b, int flags) const {
-        return kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 12 && b.fused && b.bits.size() == 5;
+        return kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 5 + kF5ColBits && b.fused && b.bits.size() == 5;
     }
 
-    // tile geometry of k_fused5_mfma: 5 target bits + the 7 lowest other bits
+    // tile geometry of k_fused5_mfma: 5 target bits + the kF5ColBits lowest other bits
     Fused5Geom fused5_geom(const std::vector<int> &tb) const {
         std::vector<std::pair<int, uint32_t>> all;
         for (int m = 0; m < 5; m++) {
-            all.emplace_back(tb[m], uint32_t{kF5Cols} << m);
+            all.emplace_back(tb[m], (1U << kF5ColBits) << m);
         }
-        for (int p = 0, m = 0; m < 7; p++) {
+        for (int p = 0, m = 0; m < kF5ColBits; p++) {
             if (std::find(tb.begin(), tb.end(), p) == tb.end()) {
                 all.emplace_back(p, 1U << m);
                 m++;
@@ -504,7 +504,7 @@ template <class T> struct Engine {
         }
         std::sort(all.begin(), all.end());
         Fused5Geom g{};
-        for (int k = 0; k < 12; k++) {
+        for (int k = 0; k < 5 + kF5ColBits; k++) {
             g.pos[k] = all[k].first;
             g.lds_of[k] = all[k].second;
         }
@@ -513,7 +513,7 @@ template <class T> struct Engine {
 
     // Applies a tape as fused blocks; returns the number of blocks.
     int apply_fused(V *sv, const TapeView &t, int max_wires, int flags, std::vector<FusedBlock> *keep) {
-        const bool probe_mfma = kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 12;
+        const bool probe_mfma = kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 5 + kF5ColBits;
         auto blocks = fuse_tape(n(), t, max_wires, probe_mfma ? 5 : 0);
         const bool renorm = flags & PL_FUSE_RENORM;
 
@@ -533,7 +533,7 @@ template <class T> struct Engine {
             }
             mbytes = (mbytes + 255) & ~size_t{255};
         }
-        const uint64_t ntiles = n() >= 12 ? (len() >> 12) : 1;
+        const uint64_t ntiles = n() >= 5 + kF5ColBits ? (len() >> (5 + kF5ColBits)) : 1;
         const size_t tail = ntiles * sizeof(double) + 256;
         std::vector<unsigned char> host(mbytes + tail, 0);
         for (size_t i = 0; i < blocks.size(); i++) {
@@ -586,13 +586,18 @@ template <class T> struct Engine {
             }
             if (mfma_ok(b, flags)) {
                 if constexpr (kMfma) {
-                    hipLaunchKernelGGL(k_fused5_mfma, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0,
-                                       h->stream, sv, ntiles, fused5_geom(b.bits),
+                    // persistent grid: the three workgroups a CU holds at once, contiguous tile ranges
+                    const uint64_t want = std::min<uint64_t>(ntiles, uint64_t(device_cus()) * 3);
+                    const uint64_t per = (ntiles + want - 1) / want;
+                    const uint64_t nblk = (ntiles + per - 1) / per; // no empty workgroup
+                    hipLaunchKernelGGL(k_fused5_mfma<kF5ColBits>, dim3(static_cast<unsigned>(nblk)),
+                                       dim3(2 << kF5ColBits), 0,
+                                       h->stream, sv, ntiles, per, fused5_geom(b.bits),
                                        reinterpret_cast<const Fused5Mat *>(dev + moff[i]),
                                        pending ? scale : nullptr, renorm ? partial : nullptr);
                     HIP_OK(hipGetLastError());
                     if (renorm) {
-                        hipLaunchKernelGGL(k_norm_scale, dim3(1), dim3(1024), 0, h->stream, partial, ntiles,
+                        hipLaunchKernelGGL(k_norm_scale, dim3(1), dim3(1024), 0, h->stream, partial, nblk,
                                            scale);
                         HIP_OK(hipGetLastError());
                         pending = true;
