@@ -95,6 +95,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
     PauliTerm *dterms = nullptr;
     uint64_t *dxm = nullptr;
     double *dparams = nullptr, *dgrads = nullptr, *denergy = nullptr;
+    double2 *dcs = nullptr; // (cos, sin) of every half angle, refreshed each replay
     int *dcirc_lists = nullptr;
     uint64_t *dprefix = nullptr;
     std::vector<LaunchSpec> seq;
@@ -152,6 +153,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
         HIP_OK(hipMalloc(&dxm, std::max<size_t>(xmasks.size(), 1) * sizeof(uint64_t)));
         HIP_OK(hipMalloc(&dparams, std::max<uint64_t>(npar, 1) * sizeof(double)));
         HIP_OK(hipMalloc(&dgrads, std::max<uint64_t>(npar, 1) * sizeof(double)));
+        HIP_OK(hipMalloc(&dcs, std::max<uint64_t>(npar, 1) * sizeof(double2)));
         HIP_OK(hipMalloc(&denergy, std::max(nc, 1) * sizeof(double)));
         if (!terms.empty()) {
             HIP_OK(hipMemcpy(dterms, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice));
@@ -173,7 +175,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
         }
         for (void *p : {static_cast<void *>(arena), static_cast<void *>(dcircs),
                         static_cast<void *>(dterms), static_cast<void *>(dxm),
-                        static_cast<void *>(dparams), static_cast<void *>(dgrads),
+                        static_cast<void *>(dparams), static_cast<void *>(dgrads), static_cast<void *>(dcs),
                         static_cast<void *>(denergy), static_cast<void *>(dcirc_lists),
                         static_cast<void *>(dprefix)}) {
             if (p) {
@@ -324,6 +326,8 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                            dgrads, npar);
         hipLaunchKernelGGL(k_zero, dim3((nc + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
                            denergy, static_cast<uint64_t>(nc));
+        hipLaunchKernelGGL(k_batch_trig, dim3((npar + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
+                           dparams, npar, dcs);
         for (const auto &ls : seq) {
             const dim3 grid(static_cast<unsigned>(ls.blocks));
             const int cur = entangler ? (ls.layer & 1) : 0;
@@ -334,7 +338,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                 break;
             case LaunchSpec::Rot:
                 hipLaunchKernelGGL((k_batch_rot<T>), grid, dim3(kBlock), tile_bytes, stream, arena,
-                                   dcircs, ls.L, W, ls.stage, ls.layer, cur, npat, pat, dparams);
+                                   dcircs, ls.L, W, ls.stage, ls.layer, cur, npat, pat, dcs);
                 break;
             case LaunchSpec::Ring:
                 hipLaunchKernelGGL((k_batch_ring<T>), grid, dim3(kBlock), 0, stream, arena, dcircs,
@@ -352,7 +356,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
             case LaunchSpec::RotAdj:
                 hipLaunchKernelGGL((k_batch_rot_adj<T>), grid, dim3(kBlock), 2 * tile_bytes, stream,
                                    arena, dcircs, ls.L, W, ls.stage, ls.layer, cur, 2 + cur, npat, pat,
-                                   dparams, dgrads);
+                                   dcs, dgrads);
                 break;
             }
             const hipError_t err = hipGetLastError();
@@ -425,7 +429,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
         HIP_OK(hipStreamSynchronize(stream));
     }
     int tile_width() const override { return W; }
-    int launches() const override { return static_cast<int>(seq.size()) + 2; }
+    int launches() const override { return static_cast<int>(seq.size()) + 3; }
     uint64_t total_params() const override { return npar; }
 };
 

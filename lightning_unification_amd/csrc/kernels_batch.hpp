@@ -94,13 +94,10 @@ template <class T> struct Rot2 {
     dcx<T> m[4];
 };
 
-// exp(-i theta/2 sigma_axis), axis 0 = X, 1 = Y, 2 = Z  (LM.hpp:541-615 matrices)
-template <class T> __device__ __forceinline__ Rot2<T> rot_matrix(int axis, double theta, bool dagger) {
-    double s, c;
-    sincos(0.5 * theta, &s, &c);
-    if (dagger) {
-        s = -s;
-    }
+// exp(-i theta/2 sigma_axis), axis 0 = X, 1 = Y, 2 = Z  (LM.hpp:541-615
+// matrices), from cs = (cos(theta/2), sin(theta/2)); dagger negates sin.
+template <class T> __device__ __forceinline__ Rot2<T> rot_matrix(int axis, double2 cs, bool dagger) {
+    const double c = cs.x, s = dagger ? -cs.y : cs.y;
     Rot2<T> r;
     if (axis == 0) {
         r.m[0] = {T(c), T(0)};
@@ -120,6 +117,19 @@ template <class T> __device__ __forceinline__ Rot2<T> rot_matrix(int axis, doubl
     }
     return r;
 }
+
+// cs[i] = (cos(theta_i / 2), sin(theta_i / 2)) for every angle of the batch,
+// once per replay, so the rotation kernels do no transcendental math
+__global__ __launch_bounds__(kBlock) void k_batch_trig(const double *__restrict__ params, uint64_t n,
+                                                      double2 *__restrict__ cs) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) {
+        double s, c;
+        sincos(0.5 * params[i], &s, &c);
+        cs[i] = double2{c, s};
+    }
+}
+
 template <class T> __device__ __forceinline__ Rot2<T> mat2_mul(const Rot2<T> &a, const Rot2<T> &b) {
     Rot2<T> r;
     for (int i = 0; i < 2; i++) {
@@ -141,13 +151,12 @@ __device__ __forceinline__ void apply2(const Rot2<T> &r, v2<T> &a, v2<T> &b) {
     b = y;
 }
 
-// local tile index e -> global amplitude index (bits of e scattered to bits[])
+// local tile index e -> global amplitude index.  Every stage's tile is bits
+// {0, 1, 2} plus one contiguous run starting at bits[3] (stage 0: the run
+// 3..T-1), so bits 0..2 of e stay in place and the rest shift to the run.
 __device__ __forceinline__ uint64_t tile_index(uint64_t base, uint32_t e, const int *bits, int T) {
-    uint64_t g = base;
-    for (int k = 0; k < T; k++) {
-        g |= static_cast<uint64_t>((e >> k) & 1U) << bits[k];
-    }
-    return g;
+    const int hlo = T > 3 ? bits[3] : 3;
+    return base | (e & 7U) | (static_cast<uint64_t>(e >> 3) << hlo);
 }
 
 // the tile's base: the block's local index spread over the non-tile bits
@@ -168,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot(v2<T> *__restrict__ arena,
                                                       const BCirc *__restrict__ circs,
                                                       const BLaunch L, int W, int stage, int layer,
                                                       int buf, int npat, int4 pat,
-                                                      const double *__restrict__ params) {
+                                                      const double2 *__restrict__ cs) {
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
     uint64_t q;
@@ -191,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot(v2<T> *__restrict__ arena,
             continue;
         }
         const int w = c.n - 1 - b;
-        const double *th = params + c.par + (static_cast<uint64_t>(layer) * c.n + w) * npat;
+        const double2 *th = cs + c.par + (static_cast<uint64_t>(layer) * c.n + w) * npat;
         Rot2<T> m = rot_matrix<T>(axes[0], th[0], false);
         for (int a = 1; a < npat; a++) {
             m = mat2_mul(rot_matrix<T>(axes[a], th[a], false), m);
@@ -322,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
                                                           const BCirc *__restrict__ circs,
                                                           const BLaunch L, int W, int stage,
                                                           int layer, int lb, int hb, int npat,
-                                                          int4 pat, const double *__restrict__ params,
+                                                          int4 pat, const double2 *__restrict__ cs,
                                                           double *__restrict__ grads) {
     extern __shared__ unsigned char smem_raw[];
     __shared__ double wpart[kBlock / 64][kBatchMaxTile][kBatchMaxPattern];
@@ -355,11 +364,24 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
         const int w = c.n - 1 - b;
         const uint64_t pidx = c.par + (static_cast<uint64_t>(layer) * c.n + w) * npat;
         double acc[kBatchMaxPattern] = {0, 0, 0, 0};
+        // letter loops run to the compile-time maximum with a guard, so the
+        // per-letter arrays keep static indices and stay in registers
+        Rot2<T> rdag[kBatchMaxPattern];
+#pragma unroll
+        for (int a = 0; a < kBatchMaxPattern; a++) {
+            if (a < npat) {
+                rdag[a] = rot_matrix<T>(axes[a], cs[pidx + a], true);
+            }
+        }
         for (uint32_t p = threadIdx.x; p < (tsz >> 1); p += kBlock) {
             const uint32_t lo = p & ((1U << k) - 1);
             const uint32_t i0 = ((p >> k) << (k + 1)) | lo, i1 = i0 | (1U << k);
             v2<T> l0 = tl[i0], l1 = tl[i1], h0 = th_[i0], h1 = th_[i1];
-            for (int a = npat - 1; a >= 0; a--) {
+#pragma unroll
+            for (int a = kBatchMaxPattern - 1; a >= 0; a--) {
+                if (a >= npat) {
+                    continue;
+                }
                 // G lambda for G = X, Y or Z on this wire
                 v2<T> g0, g1;
                 if (axes[a] == 0) {
@@ -375,19 +397,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
                 // Im(conj(h) g)
                 acc[a] += double(h0.x) * double(g0.y) - double(h0.y) * double(g0.x) +
                           double(h1.x) * double(g1.y) - double(h1.y) * double(g1.x);
-                const Rot2<T> r = rot_matrix<T>(axes[a], params[pidx + a], true);
-                apply2(r, l0, l1);
-                apply2(r, h0, h1);
+                apply2(rdag[a], l0, l1);
+                apply2(rdag[a], h0, h1);
             }
             tl[i0] = l0;
             tl[i1] = l1;
             th_[i0] = h0;
             th_[i1] = h1;
         }
-        for (int a = 0; a < npat; a++) {
-            const double s = wave_sum(acc[a]);
-            if (lane == 0) {
-                wpart[wave][slot][a] = s;
+#pragma unroll
+        for (int a = 0; a < kBatchMaxPattern; a++) {
+            if (a < npat) {
+                const double s = wave_sum(acc[a]);
+                if (lane == 0) {
+                    wpart[wave][slot][a] = s;
+                }
             }
         }
         slot++;
