@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -133,8 +134,15 @@ template <class T> struct BatchPlan final : BatchPlanBase {
             npar += static_cast<uint64_t>(d[j]) * n[j] * npat;
             term_off.push_back(static_cast<uint32_t>(terms.size()));
             term_cnt.push_back(static_cast<uint32_t>(t[j].size()));
-            terms.insert(terms.end(), t[j].begin(), t[j].end());
-            xmasks.insert(xmasks.end(), x[j].begin(), x[j].end());
+            // group the circuit's terms by x-mask: one amplitude load per mask
+            std::vector<size_t> order(t[j].size());
+            std::iota(order.begin(), order.end(), size_t{0});
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return x[j][a] < x[j][b]; });
+            for (size_t k : order) {
+                terms.push_back(t[j][k]);
+                xmasks.push_back(x[j][k]);
+            }
         }
         int dev = 0;
         HIP_OK(hipGetDevice(&dev));

@@ -151,6 +151,33 @@ __device__ __forceinline__ void apply2(const Rot2<T> &r, v2<T> &a, v2<T> &b) {
     b = y;
 }
 
+// (a, b) <- R_axis^(dagger?)(theta) (a, b) using the structure of each Pauli
+// rotation (RX symmetric with imaginary off-diagonal, RY real, RZ diagonal):
+// about half the arithmetic of a general 2x2 complex product.
+template <class T>
+__device__ __forceinline__ void rot_apply(int axis, double2 cs, bool dagger, v2<T> &a, v2<T> &b) {
+    const T c = T(cs.x), s = dagger ? T(-cs.y) : T(cs.y);
+    v2<T> x, y;
+    if (axis == 0) { // [[c, -i s], [-i s, c]]
+        x.x = c * a.x + s * b.y;
+        x.y = c * a.y - s * b.x;
+        y.x = s * a.y + c * b.x;
+        y.y = c * b.y - s * a.x;
+    } else if (axis == 1) { // [[c, -s], [s, c]]
+        x.x = c * a.x - s * b.x;
+        x.y = c * a.y - s * b.y;
+        y.x = s * a.x + c * b.x;
+        y.y = s * a.y + c * b.y;
+    } else { // diag(c - i s, c + i s)
+        x.x = c * a.x + s * a.y;
+        x.y = c * a.y - s * a.x;
+        y.x = c * b.x - s * b.y;
+        y.y = c * b.y + s * b.x;
+    }
+    a = x;
+    b = y;
+}
+
 // local tile index e -> global amplitude index.  Every stage's tile is bits
 // {0, 1, 2} plus one contiguous run starting at bits[3] (stage 0: the run
 // 3..T-1), so bits 0..2 of e stay in place and the rest shift to the run.
@@ -288,9 +315,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
     if (y < len) {
         const v2<T> *psi = arena + c.off + pbuf * len;
         double hr = 0, hi = 0;
-        for (int t = 0; t < c.nterms; t++) {
+        uint64_t xm_prev = ~uint64_t{0};
+        v2<T> a{};
+        for (int t = 0; t < c.nterms; t++) { // terms arrive grouped by x-mask
             const PauliTerm &pt = terms[c.term + t];
-            const v2<T> a = psi[y ^ xmasks[c.term + t]];
+            const uint64_t xm = xmasks[c.term + t];
+            if (xm != xm_prev) {
+                a = psi[y ^ xm];
+                xm_prev = xm;
+            }
             // coeff * (-i)^ny * (-1)^popcount(y & zy) * a
             double cr = pt.coeff, ci2 = 0;
             switch (pt.ny & 3) {
@@ -366,12 +399,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
         double acc[kBatchMaxPattern] = {0, 0, 0, 0};
         // letter loops run to the compile-time maximum with a guard, so the
         // per-letter arrays keep static indices and stay in registers
-        Rot2<T> rdag[kBatchMaxPattern];
+        double2 rcs[kBatchMaxPattern];
 #pragma unroll
         for (int a = 0; a < kBatchMaxPattern; a++) {
-            if (a < npat) {
-                rdag[a] = rot_matrix<T>(axes[a], cs[pidx + a], true);
-            }
+            rcs[a] = a < npat ? cs[pidx + a] : double2{1, 0};
         }
         for (uint32_t p = threadIdx.x; p < (tsz >> 1); p += kBlock) {
             const uint32_t lo = p & ((1U << k) - 1);
@@ -397,8 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
                 // Im(conj(h) g)
                 acc[a] += double(h0.x) * double(g0.y) - double(h0.y) * double(g0.x) +
                           double(h1.x) * double(g1.y) - double(h1.y) * double(g1.x);
-                apply2(rdag[a], l0, l1);
-                apply2(rdag[a], h0, h1);
+                rot_apply<T>(axes[a], rcs[a], true, l0, l1);
+                rot_apply<T>(axes[a], rcs[a], true, h0, h1);
             }
             tl[i0] = l0;
             tl[i1] = l1;
