@@ -257,8 +257,8 @@ int pl_batch_destroy(pl_batch_t b);
 /* A d0 -> d1 -> d2 -> d3 MLP (GELU hidden layers) trained and evaluated on the
  * GPU with BF16 matrix cores and FP32 accumulation, FP32 master weights and
  * Adam.  No reference counterpart: it screens optimiser candidates for the
- * adjoint path (AdjointJacobianLQubit.hpp:212-336 stays the judge of every
- * accepted step).  Parameters cross as FP32 in the order W1 [d0 x d1]
+ * adjoint path (the exact computation of AdjointJacobianLQubit.hpp:212-336
+ * verifies every accepted step).  Parameters cross as FP32 in the order W1 [d0 x d1]
  * row-major, b1, W2 [d1 x d2], b2, W3 [d2 x d3], b3; features as FP32 rows of
  * d0 (rounded to BF16 on the device), targets and outputs as FP32 rows of d3.
  * The loss is sum_j w_j (y_j - t_j)^2 averaged over the batch (w_j = 1 unless
