@@ -301,4 +301,106 @@ __global__ __launch_bounds__(kBlock) void k_sparse_moments(const v2<T> *__restri
     }
 }
 
+// Several x-mask groups of Pauli terms from one read of the state.  A pass
+// owns a bit set S = {0, 1, 2} plus up to kPauliTileHigh high bits; every
+// x-mask of the pass lies inside S, so both amplitudes of every pair (i, i^x)
+// are in the same tile of 2^|S| amplitudes (all combinations of S, the other
+// bits fixed).  Tiles are staged in LDS once; each group then walks its pairs
+// there.  A term's Z/Y parity splits into the tile-local bits (zyl, in the
+// compressed tile index) and the bits outside S (zyo), constant per tile.
+constexpr int kPauliTileHigh = 8;
+struct PauliTileGeom {
+    int nhigh;                    // high bits in S
+    int high[kPauliTileHigh];     // their positions, ascending (all >= 3)
+    int nlow;                     // low bits in S: min(3, n)
+};
+struct PauliTileGroup {
+    uint32_t xl; // x-mask in the compressed tile index (0: diagonal terms)
+    int t0, t1;  // its terms
+};
+struct PauliTileTerm {
+    uint64_t zyo; // Z/Y bits outside S
+    double coeff;
+    uint32_t zyl; // Z/Y bits inside S, compressed
+    int ny;
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
+                                                       const PauliTileGeom geo, int ngroups,
+                                                       const PauliTileGroup *__restrict__ groups,
+                                                       const PauliTileTerm *__restrict__ terms,
+                                                       double2 *__restrict__ partial) {
+    extern __shared__ unsigned char smem_raw[];
+    v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
+    const int nS = geo.nlow + geo.nhigh;
+    const uint32_t tsz = 1U << nS;
+    const uint32_t lowmask = (1U << geo.nlow) - 1;
+    double acc = 0;
+    for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
+        // base: q spread over the bits outside S (insert zeros at S, ascending)
+        uint64_t base = q << geo.nlow;
+        for (int k = 0; k < geo.nhigh; k++) {
+            const int p = geo.high[k];
+            base = ((base >> p) << (p + 1)) | (base & ((uint64_t{1} << p) - 1));
+        }
+        __syncthreads(); // the previous tile is no longer read
+        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+            uint64_t g = base | (e & lowmask);
+            for (int k = 0; k < geo.nhigh; k++) {
+                g |= static_cast<uint64_t>((e >> (geo.nlow + k)) & 1U) << geo.high[k];
+            }
+            tile[e] = sv[g];
+        }
+        __syncthreads();
+        for (int gi = 0; gi < ngroups; gi++) {
+            const PauliTileGroup gr = groups[gi];
+            if (gr.xl == 0) {
+                for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+                    const v2<T> a = tile[e];
+                    const double p = double(a.x) * double(a.x) + double(a.y) * double(a.y);
+                    double s = 0;
+                    for (int t = gr.t0; t < gr.t1; t++) {
+                        const PauliTileTerm tm = terms[t];
+                        const int par = (__popc(e & tm.zyl) + __popcll(base & tm.zyo)) & 1;
+                        s += par ? -tm.coeff : tm.coeff;
+                    }
+                    acc += p * s;
+                }
+                continue;
+            }
+            const int lb = __ffs(gr.xl) - 1;
+            for (uint32_t pe = threadIdx.x; pe < (tsz >> 1); pe += kBlock) {
+                const uint32_t e = ((pe >> lb) << (lb + 1)) | (pe & ((1U << lb) - 1));
+                const uint32_t f = e ^ gr.xl;
+                const v2<T> a = tile[e], c = tile[f];
+                // z = conj(a) * c
+                const double zr = double(a.x) * double(c.x) + double(a.y) * double(c.y);
+                const double zi = double(a.x) * double(c.y) - double(a.y) * double(c.x);
+                double s = 0;
+                for (int t = gr.t0; t < gr.t1; t++) {
+                    const PauliTileTerm tm = terms[t];
+                    const int po = __popcll(base & tm.zyo) & 1;
+                    const double si = ((__popc(e & tm.zyl) + po) & 1) ? -1.0 : 1.0;
+                    const double sj = ((__popc(f & tm.zyl) + po) & 1) ? -1.0 : 1.0;
+                    // Re[(-i)^ny (si z + sj conj z)]
+                    double v;
+                    switch (tm.ny & 3) {
+                    case 0: v = (si + sj) * zr; break;
+                    case 1: v = (si - sj) * zi; break;
+                    case 2: v = -(si + sj) * zr; break;
+                    default: v = -(si - sj) * zi; break;
+                    }
+                    s += tm.coeff * v;
+                }
+                acc += s;
+            }
+        }
+    }
+    const double2 r = block_sum2(double2{acc, 0});
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = r;
+    }
+}
+
 } // namespace plamd

@@ -654,43 +654,145 @@ template <class T> struct Engine {
     }
 
     // sum_t c_t <psi|P_t|psi> over Pauli words, grouped by x-mask
+    // sum_t c_t <psi|P_t|psi> over Pauli words.  Distinct x-masks are packed
+    // greedily into passes whose high bits fit one LDS tile (k_pauli_tile);
+    // each pass reads the state once.  Masks wider than a tile get a pass of
+    // their own (k_pauli_expval).
     double pauli_expval(const V *sv, const std::vector<PauliWord> &words,
                         const std::vector<double> &coeffs) {
         std::map<uint64_t, std::vector<PauliTerm>> groups;
         for (size_t t = 0; t < words.size(); t++) {
             groups[words[t].x].push_back(PauliTerm{words[t].zy, coeffs[t], words[t].ny & 3, 0});
         }
-        std::vector<PauliTerm> table;
-        std::vector<size_t> start;
+        const int nlow = std::min(3, n());
+        const uint64_t lowm = (uint64_t{1} << nlow) - 1;
+        std::vector<uint64_t> masks;
         for (auto &kv : groups) {
-            start.push_back(table.size());
-            table.insert(table.end(), kv.second.begin(), kv.second.end());
+            if (kv.first) {
+                masks.push_back(kv.first);
+            }
         }
-        auto *dterms = static_cast<PauliTerm *>(stage(h, table.data(), table.size() * sizeof(PauliTerm)));
-        const size_t ng = groups.size();
-        auto *part = reinterpret_cast<double2 *>(
-            reduce_buf(h, (ng * kReduceBlocks + 1) * sizeof(double2)));
-        size_t gi = 0;
+        std::stable_sort(masks.begin(), masks.end(), [&](uint64_t a, uint64_t b) {
+            return __builtin_popcountll(a & ~lowm) > __builtin_popcountll(b & ~lowm);
+        });
+        struct Pass {
+            uint64_t high = 0;
+            std::vector<uint64_t> xs;
+        };
+        std::vector<Pass> passes;
+        std::vector<uint64_t> wide;
+        std::vector<uint64_t> rest = masks;
+        while (!rest.empty()) {
+            Pass ps;
+            std::vector<uint64_t> left;
+            for (uint64_t x : rest) {
+                const uint64_t xh = x & ~lowm;
+                if (__builtin_popcountll(xh) > kPauliTileHigh) {
+                    wide.push_back(x);
+                } else if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh) {
+                    ps.high |= xh;
+                    ps.xs.push_back(x);
+                } else {
+                    left.push_back(x);
+                }
+            }
+            if (!ps.xs.empty()) {
+                passes.push_back(std::move(ps));
+            }
+            rest.swap(left);
+        }
+        const bool diag = groups.count(0) > 0;
+        if (diag && passes.empty()) {
+            passes.emplace_back(); // the diagonal terms alone: one pass over the state
+        }
+        // per pass: geometry, groups and compressed terms, all staged together
+        std::vector<PauliTileGeom> geos;
+        std::vector<std::pair<size_t, size_t>> gspan, tspan; // offsets into the tables
+        std::vector<PauliTileGroup> gtab;
+        std::vector<PauliTileTerm> ttab;
+        for (size_t pi = 0; pi < passes.size(); pi++) {
+            const Pass &ps = passes[pi];
+            PauliTileGeom g{};
+            g.nlow = nlow;
+            std::vector<int> pos;
+            for (int b = 0; b < nlow; b++) {
+                pos.push_back(b);
+            }
+            for (int b = 0; b < 64; b++) {
+                if ((ps.high >> b) & 1) {
+                    g.high[g.nhigh++] = b;
+                    pos.push_back(b);
+                }
+            }
+            const uint64_t smask = lowm | ps.high;
+            auto compress = [&](uint64_t v) {
+                uint32_t r = 0;
+                for (size_t k = 0; k < pos.size(); k++) {
+                    r |= static_cast<uint32_t>((v >> pos[k]) & 1) << k;
+                }
+                return r;
+            };
+            geos.push_back(g);
+            const size_t g0 = gtab.size(), t0 = ttab.size();
+            std::vector<uint64_t> xs = ps.xs;
+            if (pi == 0 && diag) {
+                xs.insert(xs.begin(), 0);
+            }
+            for (uint64_t x : xs) {
+                PauliTileGroup gr{compress(x), static_cast<int>(ttab.size()), 0};
+                for (const auto &pt : groups[x]) {
+                    ttab.push_back(PauliTileTerm{pt.zy & ~smask, pt.coeff, compress(pt.zy & smask), pt.ny});
+                }
+                gr.t1 = static_cast<int>(ttab.size());
+                gtab.push_back(gr);
+            }
+            gspan.emplace_back(g0, gtab.size() - g0);
+            tspan.emplace_back(t0, ttab.size() - t0);
+        }
+        std::vector<PauliTerm> wtab;
+        std::vector<size_t> wstart;
+        for (uint64_t x : wide) {
+            wstart.push_back(wtab.size());
+            wtab.insert(wtab.end(), groups[x].begin(), groups[x].end());
+        }
+        const size_t gb = gtab.size() * sizeof(PauliTileGroup), tb = ttab.size() * sizeof(PauliTileTerm);
+        const size_t tb_off = (gb + 255) & ~size_t{255}, wb_off = (tb_off + tb + 255) & ~size_t{255};
+        std::vector<unsigned char> blob(wb_off + wtab.size() * sizeof(PauliTerm) + 8);
+        std::memcpy(blob.data(), gtab.data(), gb);
+        std::memcpy(blob.data() + tb_off, ttab.data(), tb);
+        std::memcpy(blob.data() + wb_off, wtab.data(), wtab.size() * sizeof(PauliTerm));
+        auto *dev = static_cast<unsigned char *>(stage(h, blob.data(), blob.size()));
+        const auto *dgroups = reinterpret_cast<const PauliTileGroup *>(dev);
+        const auto *dtterms = reinterpret_cast<const PauliTileTerm *>(dev + tb_off);
+        const auto *dwterms = reinterpret_cast<const PauliTerm *>(dev + wb_off);
+        const size_t nlaunch = passes.size() + wide.size();
+        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (nlaunch * kReduceBlocks + 1) * sizeof(double2)));
         int total = 0;
-        std::vector<int> grids;
-        for (auto &kv : groups) {
-            const uint64_t x = kv.first;
-            const uint64_t nwork = x ? len() / 2 : len();
-            const int lowbit = x ? __builtin_ctzll(x) : 0;
-            const unsigned grid = std::min<unsigned>(grid_for(nwork, 1, 4), kReduceBlocks);
-            hipLaunchKernelGGL((k_pauli_expval<T>), dim3(grid), dim3(kBlock), 0, h->stream, sv,
-                               nwork, x, lowbit, static_cast<int>(kv.second.size()),
-                               dterms + start[gi], part + total);
+        for (size_t pi = 0; pi < passes.size(); pi++) {
+            const PauliTileGeom &g = geos[pi];
+            const int nS = g.nlow + g.nhigh;
+            const uint64_t ntiles = len() >> nS;
+            const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(ntiles, kReduceBlocks));
+            hipLaunchKernelGGL((k_pauli_tile<T>), dim3(grid), dim3(kBlock), (size_t{1} << nS) * sizeof(V),
+                               h->stream, sv, ntiles, g, static_cast<int>(gspan[pi].second),
+                               dgroups + gspan[pi].first, dtterms, part + total);
             HIP_OK(hipGetLastError());
             total += grid;
-            gi++;
+        }
+        for (size_t wi = 0; wi < wide.size(); wi++) {
+            const uint64_t x = wide[wi];
+            const unsigned grid = std::min<unsigned>(grid_for(len() / 2, 1, 4), kReduceBlocks);
+            hipLaunchKernelGGL((k_pauli_expval<T>), dim3(grid), dim3(kBlock), 0, h->stream, sv, len() / 2, x,
+                               __builtin_ctzll(x), static_cast<int>(groups[x].size()), dwterms + wstart[wi],
+                               part + total);
+            HIP_OK(hipGetLastError());
+            total += grid;
         }
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, total,
-                           part + ng * kReduceBlocks);
+                           part + nlaunch * kReduceBlocks);
         HIP_OK(hipGetLastError());
         double2 r;
-        HIP_OK(hipMemcpyAsync(&r, part + ng * kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost,
-                              h->stream));
+        HIP_OK(hipMemcpyAsync(&r, part + nlaunch * kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         return r.x;
     }
