@@ -163,6 +163,69 @@ def bench_batch(count, plain_budget_s=8.0, reps=3):
     return out
 
 
+def bench_c1():
+    """Plain method at the reference's CPU-runnable size: 20 qubits, 10 layers
+    of RX, RY, RZ per wire + CNOT ladder, <Z_w> for every wire and the full
+    20 x 600 adjoint Jacobian (complex128).  Reports the GPU time of forward +
+    expvals + Jacobian, the unfused-minimum algorithmic bytes of the Jacobian
+    (sum over ops of (K+1) 2 f S + per trainable op (K+1) S, plus K S) and the
+    bytes this engine's fused adjoint moves ((K+1) 2 f S per op, K + 1 state
+    preparations), and the compiled reference (all host cores, OpenMP over
+    observables as in the reference) beside the GPU on a one-layer prefix."""
+    from lightning_unification_amd import capi
+    from lightning_unification_amd.circuits import plain_layers
+    n, K = 20, 20
+    S = (1 << n) * 16
+    tape, obs, tp = plain_layers(n, 10, seed=1001)
+    sv = capi.DeviceStateVector(n, np.complex128)
+
+    def run(t, params):
+        sv.reset()
+        sv.apply_tape(t)
+        e = [sv.expval(o) for o in obs]
+        return e, sv.adjoint_jacobian(t, obs, params)
+
+    run(tape, tp)  # warm
+    t0 = time.perf_counter()
+    _, jac = run(tape, tp)
+    t_gpu = time.perf_counter() - t0
+    alg = sum((K + 1) * 2 * touched_fraction(op[0]) * S for op in tape.ops) + len(tp) * (K + 1) * S + K * S
+    moved = sum((K + 1) * 2 * touched_fraction(op[0]) * S for op in tape.ops) + (K + 1) * 2 * S
+    out = {"qubits": n, "ops": len(tape), "params": len(tp), "observables": K,
+           "gpu_forward_expval_jacobian_ms": round(t_gpu * 1e3, 1),
+           "jacobian_algorithmic_GB": round(alg / 1e9, 1),
+           "jacobian_moved_GB": round(moved / 1e9, 1),
+           "algorithmic_GBs": round(alg / t_gpu / 1e9, 1)}
+    # one-layer prefix on both sides
+    pre, _, _ = plain_layers(n, 1, seed=1001)
+    ptp = list(range(pre.num_params()))
+    t0 = time.perf_counter()
+    run(pre, ptp)
+    t_gpu1 = time.perf_counter() - t0
+    try:
+        from oracle.pyoracle import Oracle, REF_SO
+        kind = "reference" if os.path.exists(REF_SO) else "port"
+        orc = Oracle(ref=(kind == "reference"))
+        s = np.zeros(1 << n, np.complex128)
+        s[0] = 1
+        t0 = time.perf_counter()
+        orc.apply_tape(s, pre)
+        ref_e = [orc.expval(s, o) for o in obs]
+        ref_j = orc.adjoint(s, pre, obs, ptp)
+        t_cpu1 = time.perf_counter() - t0
+        _, gj = run(pre, ptp)
+        out["one_layer_prefix"] = {"ops": len(pre), "params": len(ptp), "gpu_ms": round(t_gpu1 * 1e3, 1),
+                                   "cpu_ms": round(t_cpu1 * 1e3, 1), "cpu_kind": kind,
+                                   "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                                   "max_abs_jacobian_diff": float(np.max(np.abs(gj - ref_j)))}
+        del ref_e
+    except Exception as e:  # oracle not built
+        log(f"c1 cpu leg unavailable: {e}")
+    sv.close()
+    log(f"C1: forward + 20 expvals + 20x600 Jacobian in {t_gpu*1e3:.1f} ms")
+    return out
+
+
 def bench_precision(n_big=33, n_ref=30):
     """Precision form: the same seeded 50-layer gate mix, consecutive gates
     fused into 5-qubit 32x32 blocks applied to a complex64 state on FP32
@@ -286,7 +349,8 @@ def main():
                     help="qubits of the complex64 fused-block run (0 skips the precision form)")
     ap.add_argument("--surrogate", type=int, default=1 << 20,
                     help="training tuples of the learned-surrogate form (0 skips it)")
-    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate"],
+    ap.add_argument("--no-c1", action="store_true", help="skip the 20-qubit adjoint measurement")
+    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate", "c1"],
                     default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
@@ -310,6 +374,9 @@ def main():
         return
     if args.mode == "precision":
         print(json.dumps({"mode": "precision", "precision": bench_precision(args.precision)}))
+        return
+    if args.mode == "c1":
+        print(json.dumps({"mode": "c1", "c1": bench_c1()}))
         return
     if args.mode == "surrogate":
         print(json.dumps({"mode": "surrogate", "surrogate": bench_surrogate(args.surrogate)}))
@@ -381,6 +448,7 @@ def main():
     log(f"<H> = {e:.12f} ({t_expval*1e3:.1f} ms), {len(idx)} samples ({t_sample*1e3:.1f} ms)")
     sv.close()
 
+    c1 = None if args.no_c1 else bench_c1()
     batched = bench_batch(args.batch, args.plain_s) if args.batch > 0 else None
     precision = bench_precision(args.precision) if args.precision > 0 else None
     surrogate = bench_surrogate(args.surrogate) if args.surrogate > 0 else None
@@ -412,6 +480,7 @@ def main():
         "per_gate_GBs": per_gate_gbs,
         "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},
         "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
+        "c1_adjoint": c1,
         "batched": batched,
         "precision": precision,
         "surrogate": surrogate,

@@ -26,6 +26,7 @@ template <class T, int M> struct AdjOp {
 };
 
 constexpr int kAdjMaxObs = 16; // observables per launch
+constexpr int kAdjChunk = 4;    // observables whose loads are in flight together
 
 template <class T, int M>
 __device__ __forceinline__ void matvec(const dcx<T> *m, int diag, const v2<T> *x, v2<T> *y) {
@@ -44,6 +45,31 @@ __device__ __forceinline__ void matvec(const dcx<T> *m, int diag, const v2<T> *x
             }
             y[r] = acc;
         }
+    }
+}
+
+// Block totals of the per-thread accumulators acc[0 .. nobs): each wave sums
+// with shuffles into an LDS table, one barrier, then thread k adds the waves'
+// entries in order (nobs is uniform across the block).
+__device__ __forceinline__ void adj_block_totals(const double *acc, int nobs, double *__restrict__ out) {
+    __shared__ double wpart[kBlock / 64][kAdjMaxObs];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kAdjMaxObs; k++) {
+        if (k < nobs) {
+            const double s = wave_sum(acc[k]);
+            if (lane == 0) {
+                wpart[wave][k] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < nobs) {
+        double t = 0;
+        for (int w = 0; w < kBlock / 64; w++) {
+            t += wpart[w][threadIdx.x];
+        }
+        out[threadIdx.x] = t;
     }
 }
 
@@ -82,41 +108,45 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_step(v2<T> *const *__restric
                 lam[base | op.off[j]] = y[j];
             }
         }
+        // observables in chunks: a chunk's loads are all issued before its
+        // stores (the H_k may alias as far as the compiler knows)
 #pragma unroll
-        for (int k = 0; k < kAdjMaxObs; k++) {
-            if (k < nobs) {
-                v2<T> *h = states[1 + k];
+        for (int k0 = 0; k0 < kAdjMaxObs; k0 += kAdjChunk) {
+            v2<T> hx[kAdjChunk][M];
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    x[j] = h[base | op.off[j]];
-                }
-                if (TRAIN) {
-                    double s = 0;
+            for (int c = 0; c < kAdjChunk; c++) {
+                if (k0 + c < nobs) {
+                    const v2<T> *h = states[1 + k0 + c];
 #pragma unroll
                     for (int j = 0; j < M; j++) {
-                        // Im(conj(h) * mu)
-                        s += double(x[j].x) * double(mu[j].y) - double(x[j].y) * double(mu[j].x);
+                        hx[c][j] = h[base | op.off[j]];
                     }
-                    acc[k] += s;
                 }
-                matvec<T, M>(op.ud, op.ud_diag, x, y);
+            }
 #pragma unroll
-                for (int j = 0; j < M; j++) {
-                    h[base | op.off[j]] = y[j];
+            for (int c = 0; c < kAdjChunk; c++) {
+                if (k0 + c < nobs) {
+                    if (TRAIN) {
+                        double s = 0;
+#pragma unroll
+                        for (int j = 0; j < M; j++) {
+                            // Im(conj(h) * mu)
+                            s += double(hx[c][j].x) * double(mu[j].y) - double(hx[c][j].y) * double(mu[j].x);
+                        }
+                        acc[k0 + c] += s;
+                    }
+                    matvec<T, M>(op.ud, op.ud_diag, hx[c], y);
+                    v2<T> *h = states[1 + k0 + c];
+#pragma unroll
+                    for (int j = 0; j < M; j++) {
+                        h[base | op.off[j]] = y[j];
+                    }
                 }
             }
         }
     }
     if (TRAIN) {
-#pragma unroll
-        for (int k = 0; k < kAdjMaxObs; k++) {
-            if (k < nobs) { // nobs is uniform: every thread reaches the barrier
-                const double2 r = block_sum2(double2{acc[k], 0});
-                if (threadIdx.x == 0) {
-                    partial[static_cast<uint64_t>(blockIdx.x) * nobs + k] = r.x;
-                }
-            }
-        }
+        adj_block_totals(acc, nobs, partial + static_cast<uint64_t>(blockIdx.x) * nobs);
     }
 }
 
@@ -150,40 +180,42 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_parity(v2<T> *const *__restr
             lam[i] = cmul(u, x);
         }
 #pragma unroll
-        for (int k = 0; k < kAdjMaxObs; k++) {
-            if (k < nobs) {
-                v2<T> *h = states[1 + k];
-                const v2<T> hv = h[i];
-                if (TRAIN) {
-                    acc[k] += double(hv.x) * double(mu.y) - double(hv.y) * double(mu.x);
+        for (int k0 = 0; k0 < kAdjMaxObs; k0 += kAdjChunk) {
+            v2<T> hv[kAdjChunk];
+#pragma unroll
+            for (int c = 0; c < kAdjChunk; c++) {
+                if (k0 + c < nobs) {
+                    hv[c] = states[1 + k0 + c][i];
                 }
-                h[i] = cmul(u, hv);
+            }
+#pragma unroll
+            for (int c = 0; c < kAdjChunk; c++) {
+                if (k0 + c < nobs) {
+                    if (TRAIN) {
+                        acc[k0 + c] += double(hv[c].x) * double(mu.y) - double(hv[c].y) * double(mu.x);
+                    }
+                    states[1 + k0 + c][i] = cmul(u, hv[c]);
+                }
             }
         }
     }
     if (TRAIN) {
-#pragma unroll
-        for (int k = 0; k < kAdjMaxObs; k++) {
-            if (k < nobs) { // nobs is uniform: every thread reaches the barrier
-                const double2 r = block_sum2(double2{acc[k], 0});
-                if (threadIdx.x == 0) {
-                    partial[static_cast<uint64_t>(blockIdx.x) * nobs + k] = r.x;
-                }
-            }
-        }
+        adj_block_totals(acc, nobs, partial + static_cast<uint64_t>(blockIdx.x) * nobs);
     }
 }
 
-// jac[row + k] = factor * sum_b partial[b * nobs + k]   (one thread per k)
-__global__ void k_adjoint_final(const double *__restrict__ partial, int nblocks, int nobs,
-                                double factor, double *__restrict__ jac) {
-    const int k = threadIdx.x;
-    if (k < nobs) {
-        double s = 0;
-        for (int b = 0; b < nblocks; b++) {
-            s += partial[b * nobs + k];
-        }
-        jac[k] = factor * s;
+// jac[row + k] = factor * sum_b partial[b * nobs + k]: block k, 256 threads
+// each summing a strided share of the blocks, then a fixed-order block sum
+__global__ __launch_bounds__(kBlock) void k_adjoint_final(const double *__restrict__ partial, int nblocks,
+                                                          int nobs, double factor, double *__restrict__ jac) {
+    const int k = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+        s += partial[static_cast<size_t>(b) * nobs + k];
+    }
+    const double2 r = block_sum2(double2{s, 0});
+    if (threadIdx.x == 0) {
+        jac[k] = factor * r.x;
     }
 }
 

@@ -1227,7 +1227,17 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
     V **dstates = nullptr;
     double *djac = nullptr, *dpart = nullptr;
     const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 8), kReduceBlocks);
-    HIP_OK(hipMalloc(&dstates, states.size() * sizeof(V *)));
+    // one pointer table per chunk of kAdjMaxObs observables: lambda, then the
+    // chunk's H_k (built once, so no launch waits on a staging copy)
+    const int nchunk = (K + kAdjMaxObs - 1) / kAdjMaxObs;
+    std::vector<V *> tables(size_t(nchunk) * (kAdjMaxObs + 1), nullptr);
+    for (int c = 0; c < nchunk; c++) {
+        tables[size_t(c) * (kAdjMaxObs + 1)] = lam;
+        for (int k = c * kAdjMaxObs; k < std::min(K, (c + 1) * kAdjMaxObs); k++) {
+            tables[size_t(c) * (kAdjMaxObs + 1) + 1 + (k - c * kAdjMaxObs)] = states[1 + k];
+        }
+    }
+    HIP_OK(hipMalloc(&dstates, tables.size() * sizeof(V *)));
     HIP_OK(hipMalloc(&djac, size_t(ntp) * K * sizeof(double)));
     HIP_OK(hipMalloc(&dpart, size_t(grid) * kAdjMaxObs * sizeof(double)));
     struct Free {
@@ -1238,7 +1248,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
             (void)hipFree(c);
         }
     } fr{dstates, djac, dpart};
-    HIP_OK(hipMemcpyAsync(dstates, states.data(), states.size() * sizeof(V *),
+    HIP_OK(hipMemcpyAsync(dstates, tables.data(), tables.size() * sizeof(V *),
                           hipMemcpyHostToDevice, E.h->stream));
     HIP_OK(hipMemsetAsync(djac, 0, size_t(ntp) * K * sizeof(double), E.h->stream));
 
@@ -1322,16 +1332,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
             const int nk = std::min(kAdjMaxObs, K - k0);
             const int upd = (k0 + nk == K) ? 1 : 0;
             // states for this launch: lambda followed by H_k0 .. H_k0+nk-1
-            V *const *st = dstates;
-            std::vector<V *> local;
-            if (k0 > 0) {
-                // device pointer table is contiguous: shift by k0 and put lambda first
-                local.push_back(lam);
-                for (int k = 0; k < nk; k++) {
-                    local.push_back(states[1 + k0 + k]);
-                }
-                st = static_cast<V *const *>(stage(h, local.data(), local.size() * sizeof(V *)));
-            }
+            V *const *st = dstates + size_t(k0 / kAdjMaxObs) * (kAdjMaxObs + 1);
             if (ud.kind == GroupOp<T>::Parity) {
                 const dcx<T> u0{ud.parity_phase[0].real(), ud.parity_phase[0].imag()};
                 const dcx<T> u1{ud.parity_phase[1].real(), ud.parity_phase[1].imag()};
@@ -1383,7 +1384,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                 }
                 if (train) {
                     HIP_OK(hipGetLastError());
-                    hipLaunchKernelGGL(k_adjoint_final, dim3(1), dim3(64), 0, E.h->stream, dpart,
+                    hipLaunchKernelGGL(k_adjoint_final, dim3(nk), dim3(kBlock), 0, E.h->stream, dpart,
                                        (int)gr, nk, factor, djac + row + k0);
                 }
                 HIP_OK(hipGetLastError());
@@ -1391,7 +1392,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
             }
             HIP_OK(hipGetLastError());
             if (train) {
-                hipLaunchKernelGGL(k_adjoint_final, dim3(1), dim3(64), 0, E.h->stream, dpart,
+                hipLaunchKernelGGL(k_adjoint_final, dim3(nk), dim3(kBlock), 0, E.h->stream, dpart,
                                    (int)grid, nk, factor, djac + row + k0);
                 HIP_OK(hipGetLastError());
             }
