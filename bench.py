@@ -95,7 +95,7 @@ def cpu_baseline(tape, n, budget_s=20.0):
                       f"{t_total:.1f} s single-threaded"}
 
 
-def bench_batch(count, plain_budget_s=8.0, reps=3):
+def bench_batch(count, plain_budget_s=8.0, reps=3, cpu_budget_s=10.0):
     """Batching form: `count` HEA circuits (8..20 qubits, depth 4..12, RY+RZ
     + CNOT ring, periodic TFIM each), energy + full gradient for every circuit.
     Batched: one hipGraph replay of pl_batch_run per step (all circuits).
@@ -160,6 +160,34 @@ def bench_batch(count, plain_budget_s=8.0, reps=3):
     out["speedup_c128"] = round(out["c128"]["grads_per_s"] / plain_gps, 1)
     out["speedup_c64"] = round(out["c64"]["grads_per_s"] / plain_gps, 1)
     log(f"plain: {done} circuits in {t_plain:.1f} s; speedup c128 {out['speedup_c128']}x")
+
+    # the reference's CPU path on the same seeded sample (forward, expval,
+    # adjoint with one observable), bounded by cpu_budget_s
+    try:
+        from oracle.pyoracle import Oracle, REF_SO
+        kind = "reference" if os.path.exists(REF_SO) else "port"
+        orc = Oracle(ref=(kind == "reference"))
+        done_c, t_cpu = 0, 0.0
+        for j in Rng(5).shuffle(range(count)):
+            n, d = ns[j], ds[j]
+            p = params[offs[j]:offs[j + 1]]
+            tape = layered_tape(n, d, "YZ", "ring", p)
+            t0 = time.perf_counter()
+            s = np.zeros(1 << n, np.complex128)
+            s[0] = 1
+            orc.apply_tape(s, tape)
+            orc.expval(s, obs[j])
+            orc.adjoint(s, tape, [obs[j]], list(range(p.size)))
+            t_cpu += time.perf_counter() - t0
+            done_c += 1
+            if t_cpu > cpu_budget_s:
+                break
+        out["cpu_reference_c128"] = {"grads_per_s": round(done_c / t_cpu, 2), "sample_circuits": done_c,
+                                     "sample_s": round(t_cpu, 2), "kind": kind,
+                                     "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))}
+        log(f"cpu reference: {done_c} circuits in {t_cpu:.1f} s")
+    except Exception as e:  # oracle not built
+        log(f"batch cpu leg unavailable: {e}")
     return out
 
 
