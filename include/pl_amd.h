@@ -213,6 +213,34 @@ int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
                  const double *params, int max_wires, int pad_to, int capacity,
                  int *nblocks, int *first_op, int *block_nops, int *nbits,
                  int *bits, double *mats);
+/* StateVectorBase::applyOperations (StateVectorBase.hpp:84-120) executed as
+ * tile passes: the tape is split into passes, each of which loads tiles of
+ * 2^tile_bits amplitudes (the low_bits lowest bit positions plus
+ * tile_bits - low_bits higher ones) into LDS, applies every operation of the
+ * pass there and writes the tiles back, so the state is read and written once
+ * per pass.  Operations move ahead of earlier ones only across operations on
+ * disjoint wires, with which they commute.  Operations a pass cannot hold are
+ * applied by the per-gate kernels.  tile_bits in [9, 12] and <= the qubit
+ * count; low_bits in [0, 6].  tile_bits = 0 and low_bits = -1 pick the
+ * defaults of the precision (2^10 amplitudes and 3 low bits for complex128,
+ * 2^11 and 4 for complex64, clamped to the state); with tile_bits = 0 a state
+ * of fewer than 9 qubits runs gate by gate.  *npasses (may be NULL) receives
+ * the number of launches. */
+int pl_sv_apply_ops_tiled(pl_sv_t sv, int nops, const int *ids, const int *nwires,
+                          const int *wires, const uint8_t *inverse,
+                          const int *nparams, const double *params, int tile_bits,
+                          int low_bits, int *npasses);
+/* The pass plan of pl_sv_apply_ops_tiled, computed on the host only: for
+ * pass p, pass_nops[p] operations, pass_tiled[p] (0 = one operation applied
+ * by the per-gate kernels), pass_bits[16 p ..] the higher tile bit positions
+ * (ascending, -1 padded); order[] lists the tape indices of all operations
+ * (state preparation excluded) in application order, pass after pass.
+ * Fails with PL_ERR_INVALID, and sets *npasses, when capacity is too small. */
+int pl_tile_plan(int num_qubits, int nops, const int *ids, const int *nwires,
+                 const int *wires, const uint8_t *inverse, const int *nparams,
+                 const double *params, int tile_bits, int low_bits, int capacity,
+                 int *npasses, int *pass_nops, int *pass_tiled, int *pass_bits,
+                 int *order);
 /* <psi|psi> accumulated in FP64 (innerProdC(psi, psi), LinearAlgebra.hpp:193-213). */
 int pl_sv_norm2(pl_sv_t sv, double *out);
 /* ||a - b||_2 accumulated in FP64; a and b must have the same qubit count and

@@ -31,7 +31,7 @@ EXPORTS = [
     "pl_sv_norm2", "pl_sv_distance", "pl_mlp_create", "pl_mlp_destroy", "pl_mlp_num_params",
     "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
     "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train", "pl_vjp", "pl_expval_sparse",
-    "pl_var_sparse", "pl_generate_mcmc_samples",
+    "pl_var_sparse", "pl_generate_mcmc_samples", "pl_sv_apply_ops_tiled", "pl_tile_plan",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -192,6 +192,18 @@ class DeviceStateVector:
             _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)), _p(_f64(par)),
             int(max_wires), flags, ctypes.byref(nb)))
         return nb.value
+
+    def apply_tape_tiled(self, tape, tile_bits=0, low_bits=-1):
+        """Apply a tape as tile passes (pl_sv_apply_ops_tiled); returns the
+        number of launches.  tile_bits=0 / low_bits=-1: the defaults of the
+        precision."""
+        ids, nw, wires, inv, npar, par = tape.encode()
+        npass = ctypes.c_int()
+        _check(load().pl_sv_apply_ops_tiled(
+            self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+            _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)), _p(_f64(par)),
+            int(tile_bits), int(low_bits), ctypes.byref(npass)))
+        return npass.value
 
     def norm2(self):
         out = ctypes.c_double()
@@ -359,6 +371,27 @@ def fuse_tape(num_qubits, tape, max_wires=5, pad_to=0):
         d = 1 << k
         m = mats[2048 * b: 2048 * b + 2 * d * d].view(np.complex128).reshape(d, d).copy()
         out.append((int(first[b]), int(nops[b]), bits[8 * b: 8 * b + k].tolist(), m))
+    return out
+
+
+def tile_plan(num_qubits, tape, tile_bits=12, low_bits=6):
+    """Host-only pass plan of pl_sv_apply_ops_tiled: list of passes
+    (tiled, high tile bits, tape op indices in application order)."""
+    ids, nw, wires, inv, npar, par = tape.encode()
+    cap = max(1, len(ids))
+    npass = ctypes.c_int()
+    pn, pt = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    pb = np.zeros(16 * cap, np.int32)
+    order = np.zeros(cap, np.int32)
+    _check(load().pl_tile_plan(int(num_qubits), len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
+                               _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)),
+                               _p(_f64(par)), int(tile_bits), int(low_bits), cap, ctypes.byref(npass),
+                               _p(pn), _p(pt), _p(pb), _p(order)))
+    out, at = [], 0
+    for p in range(npass.value):
+        hb = [int(b) for b in pb[16 * p: 16 * p + 16] if b >= 0]
+        out.append((bool(pt[p]), hb, order[at: at + pn[p]].tolist()))
+        at += int(pn[p])
     return out
 
 

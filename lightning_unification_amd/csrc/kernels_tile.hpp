@@ -1,0 +1,564 @@
+// Multi-gate tile passes: cache blocking of a gate sequence on chip.
+//
+// A pass fixes a set of k "tile bits": the `low` lowest bit positions
+// (contiguous runs of 2^low amplitudes in HBM) plus k - low higher positions
+// hb[0] < hb[1] < ...  Every assignment of the other n - k bits selects one
+// tile of 2^k amplitudes (local index e: bits 0..low-1 of e are the
+// amplitude's bits 0..low-1, bit low + j of e is its bit hb[j]).  One
+// wavefront loads a tile, applies every operation of the pass -- each lowered
+// to a group op on the k-qubit local system, so the arithmetic is the
+// per-gate kernels' -- and writes the tile back.  HBM sees one read and one
+// write of the state per pass, however many gates the pass holds.
+//
+// The tile lives in registers: lane l holds v[r], r < R = 2^(k-6), the
+// amplitude of local index l | r << 6.  Lane bits are local bits 0..5 (so a
+// wavefront load covers runs of 2^low amplitudes), register bits are local
+// bits 6..k-1.  Operations of a pass, compiled on the host:
+//   diag     multiply every amplitude by the phase its index bits select;
+//   parity   the same for a parity phase (MultiRZ);
+//   p1/p2/p4 a group op whose member bits are register bits: each lane
+//            updates its own registers, indices fixed at compile time;
+//   lane     a 2x2 op whose member bit is a lane bit: one cross-lane
+//            exchange per register;
+//   lds      any other op, applied by tp_group / tp_parity through LDS
+//            (the launch then carries a tile of LDS).
+// Control bits may be lane bits (a per-lane predicate) or register bits (a
+// uniform predicate on the compile-time register index).  No workgroup
+// barrier is needed outside the lds case: the wavefront is the workgroup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels_gate.hpp"
+
+namespace plamd {
+
+constexpr int kTpLanes = 64;   // one wavefront per tile
+constexpr int kTpMinBits = 9;  // tile = 2^k amplitudes, k in [9, 12]
+constexpr int kTpMaxBits = 12;
+constexpr int kTpMaxLow = 6;   // the lane index spans local bits 0..5
+constexpr int kTpMaxM = 8;     // amplitudes per group of an lds op
+constexpr int kTpMaxPins = 6;  // pinned positions of a tiled op
+
+template <class T> struct TpOp {
+    int kind;                   // 0 group, 1 diagonal group, 2 parity phase
+    int M;                      // 1, 2, 4 or 8 amplitudes per group
+    int nb;                     // pinned local positions
+    int pos[kTpMaxPins];        // ascending
+    uint32_t fixed;             // local bits forced to 1 in every member
+    uint32_t mask;              // parity: local bits whose parity picks the phase
+    uint32_t off[kTpMaxM];      // member offsets (local)
+    dcx<T> mat[kTpMaxM * kTpMaxM]; // M x M row-major; parity: [0] even, [1] odd
+};
+
+struct TpGeom {
+    int k;                 // tile bits
+    int low;               // local bits 0..low-1 = amplitude bits 0..low-1
+    int hb[kTpMaxBits];    // amplitude bit of local bit low + j (ascending)
+    Geom outer;            // pins hb[]: tile index << low -> base index
+};
+
+template <class T, int M>
+__device__ __forceinline__ void tp_group(v2<T> *tile, const TpOp<T> &op, int k, int tid) {
+    const int ng = (1 << k) >> op.nb;
+    uint32_t off[M];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        off[j] = op.off[j];
+    }
+    const bool diag = op.kind == 1;
+    for (int q = tid; q < ng; q += kTpLanes) {
+        uint32_t be = static_cast<uint32_t>(q);
+        for (int j = 0; j < op.nb; j++) {
+            const int p = op.pos[j];
+            be = ((be >> p) << (p + 1)) | (be & ((1U << p) - 1));
+        }
+        be |= op.fixed;
+        v2<T> v[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            v[j] = tile[be | off[j]];
+        }
+        if (diag) {
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                tile[be | off[j]] = cmul(op.mat[j * M + j], v[j]);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < M; r++) {
+                v2<T> acc = cmul(op.mat[r * M], v[0]);
+#pragma unroll
+                for (int c = 1; c < M; c++) {
+                    cfma(acc, op.mat[r * M + c], v[c]);
+                }
+                tile[be | off[r]] = acc;
+            }
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void tp_parity(v2<T> *tile, const TpOp<T> &op, int k, int tid) {
+    const dcx<T> p0 = op.mat[0], p1 = op.mat[1];
+    for (int e = tid; e < (1 << k); e += kTpLanes) {
+        tile[e] = cmul((__popc(static_cast<uint32_t>(e) & op.mask) & 1) ? p1 : p0, tile[e]);
+    }
+}
+
+// v from lane `src` (ds_bpermute on 32-bit words)
+__device__ __forceinline__ float tp_bperm(float x, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+__device__ __forceinline__ double tp_bperm(double x, int src) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b));
+    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <class T> __device__ __forceinline__ v2<T> tp_lane_xor(const v2<T> &v, int src) {
+    v2<T> o;
+    o.x = tp_bperm(v.x, src);
+    o.y = tp_bperm(v.y, src);
+    return o;
+}
+
+enum TrCode : int {
+    kTrDiag = 1, kTrParity = 2, kTrP1 = 3, kTrP2 = 4, kTrP4 = 5, kTrLds = 6, kTrLane = 7, kTrMix1 = 8, kTrMix2 = 9
+};
+constexpr int kTrMaxKB = kTpMaxBits - 6;
+
+template <class T> struct TrInst {
+    int code;
+    int ja, jb;              // p1: ja; p2/p4/diag: register bits ja < jb
+    uint32_t rmask, rval;    // register-index condition; parity: register-index parity mask
+    uint32_t tmask, tval;    // lane-index condition; parity: lane-index parity mask
+    uint32_t lx, lx2;        // lane / mix1: the lane member bit; mix2: both, as lane-index masks
+    int nr;                  // diag: member bits held in registers (0..2)
+    int rcb[2];              // diag: phase-index bit fed by register ja / jb
+    int tcb[2];              // diag: lane bit feeding phase-index bit i, or -1
+    int lds_op;              // lds: index into the TpOp table
+    dcx<T> mat[16];          // p1/p2/lane: 2x2; p4: 4x4; diag: phase by index c
+                             // (bit i = member bit i); parity: even, odd;
+                             // mix1: [8 o + 4 j + c] row j of lanes whose lane
+                             // member bit is o, over (own j=0, own j=1,
+                             // partner j=0, partner j=1); mix2: [4 o + d] the
+                             // row of lanes with member pattern o, column o ^ d
+};
+
+// Complex products with every real operation fused: a * x in 4 operations,
+// a * x + b * y in 8.
+template <class T> __device__ __forceinline__ v2<T> tp_mul(dcx<T> a, v2<T> x) {
+    v2<T> o;
+    o.x = fma(a.r, x.x, -a.i * x.y);
+    o.y = fma(a.r, x.y, a.i * x.x);
+    return o;
+}
+template <class T> __device__ __forceinline__ void tp_mac(v2<T> &acc, dcx<T> a, v2<T> x) {
+    acc.x = fma(a.r, x.x, fma(-a.i, x.y, acc.x));
+    acc.y = fma(a.r, x.y, fma(a.i, x.x, acc.y));
+}
+template <class T> __device__ __forceinline__ v2<T> tp_mul2(dcx<T> a, v2<T> x, dcx<T> b, v2<T> y) {
+    v2<T> o = tp_mul(b, y);
+    tp_mac(o, a, x);
+    return o;
+}
+
+// CHK: the op has register-bit controls (a uniform test per register group)
+template <class T, int KB, int J, bool CHK>
+__device__ __forceinline__ void tr_p1(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+    const dcx<T> m00 = in.mat[0], m01 = in.mat[1], m10 = in.mat[2], m11 = in.mat[3];
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if ((r >> J) & 1) {
+            continue;
+        }
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        constexpr int b = 1 << J;
+        const v2<T> a = v[r], c = v[r | b];
+        v[r] = tp_mul2(m00, a, m01, c);
+        v[r | b] = tp_mul2(m10, a, m11, c);
+    }
+}
+
+// members r | 1 << JA and r | 1 << JB (bits JA, JB of r clear)
+template <class T, int KB, int JA, int JB, bool CHK>
+__device__ __forceinline__ void tr_p2(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+    const dcx<T> m00 = in.mat[0], m01 = in.mat[1], m10 = in.mat[2], m11 = in.mat[3];
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if (((r >> JA) & 1) || ((r >> JB) & 1)) {
+            continue;
+        }
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        constexpr int a_ = 1 << JA, b_ = 1 << JB;
+        const v2<T> a = v[r | a_], c = v[r | b_];
+        v[r | a_] = tp_mul2(m00, a, m01, c);
+        v[r | b_] = tp_mul2(m10, a, m11, c);
+    }
+}
+
+// members r, r | 1 << JA, r | 1 << JB, r | both
+template <class T, int KB, int JA, int JB, bool CHK>
+__device__ __forceinline__ void tr_p4(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if (((r >> JA) & 1) || ((r >> JB) & 1)) {
+            continue;
+        }
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        constexpr int a_ = 1 << JA, b_ = 1 << JB;
+        const v2<T> x[4] = {v[r], v[r | a_], v[r | b_], v[r | a_ | b_]};
+        v2<T> y[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            y[i] = tp_mul(in.mat[4 * i], x[0]);
+#pragma unroll
+            for (int c = 1; c < 4; c++) {
+                tp_mac(y[i], in.mat[4 * i + c], x[c]);
+            }
+        }
+        v[r] = y[0];
+        v[r | a_] = y[1];
+        v[r | b_] = y[2];
+        v[r | a_ | b_] = y[3];
+    }
+}
+
+// One independent test per register bit (no else chain): each case keeps
+// its compile-time register indices.
+// diagonal op: v[r] *= pt[c_r], c_r = the phase-index bits register r feeds
+template <class T, int KB, int NR, int J0, int J1>
+__device__ __forceinline__ void tr_diag(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        int c = 0;
+        if constexpr (NR >= 1) {
+            c |= (r >> J0) & 1;
+        }
+        if constexpr (NR >= 2) {
+            c |= ((r >> J1) & 1) << 1;
+        }
+        v[r] = tp_mul(pt[c], v[r]);
+    }
+}
+
+template <class T, int KB, int J = 0>
+__device__ __forceinline__ void tr_diag1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+    if constexpr (J < KB) {
+        if (in.ja == J) {
+            tr_diag<T, KB, 1, J, 0>(v, in, pt);
+        }
+        tr_diag1_at<T, KB, J + 1>(v, in, pt);
+    }
+}
+
+template <class T, int KB, int JA = 0, int JB = 1>
+__device__ __forceinline__ void tr_diag2_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+    if constexpr (JB < KB) {
+        if (in.ja == JA && in.jb == JB) {
+            tr_diag<T, KB, 2, JA, JB>(v, in, pt);
+        }
+        if constexpr (JA + 1 < JB) {
+            tr_diag2_at<T, KB, JA + 1, JB>(v, in, pt);
+        } else {
+            tr_diag2_at<T, KB, 0, JB + 1>(v, in, pt);
+        }
+    }
+}
+
+template <class T, int KB, bool CHK, int J = 0>
+__device__ __forceinline__ void tr_p1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+    if constexpr (J < KB) {
+        if (in.ja == J) {
+            tr_p1<T, KB, J, CHK>(v, in);
+        }
+        tr_p1_at<T, KB, CHK, J + 1>(v, in);
+    }
+}
+
+template <class T, int KB, bool P4, bool CHK, int JA = 0, int JB = 1>
+__device__ __forceinline__ void tr_pair_at(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+    if constexpr (JB < KB) {
+        if (in.ja == JA && in.jb == JB) {
+            if constexpr (P4) {
+                tr_p4<T, KB, JA, JB, CHK>(v, in);
+            } else {
+                tr_p2<T, KB, JA, JB, CHK>(v, in);
+            }
+        }
+        if constexpr (JA + 1 < JB) {
+            tr_pair_at<T, KB, P4, CHK, JA + 1, JB>(v, in);
+        } else {
+            tr_pair_at<T, KB, P4, CHK, 0, JB + 1>(v, in);
+        }
+    }
+}
+
+// one lane member bit (lx) and one register member bit J: per register pair
+// a lane exchanges both amplitudes with lane ^ lx and forms its own two
+template <class T, int KB, int J, bool CHK>
+__device__ __forceinline__ void tr_mix1(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
+                                        int src, bool on) {
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if ((r >> J) & 1) {
+            continue;
+        }
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        constexpr int b = 1 << J;
+        const v2<T> a0 = v[r], a1 = v[r | b];
+        const v2<T> p0 = tp_lane_xor<T>(a0, src), p1 = tp_lane_xor<T>(a1, src);
+        v2<T> y0 = tp_mul(w[0], a0), y1 = tp_mul(w[4], a0);
+        tp_mac(y0, w[1], a1);
+        tp_mac(y1, w[5], a1);
+        tp_mac(y0, w[2], p0);
+        tp_mac(y1, w[6], p0);
+        tp_mac(y0, w[3], p1);
+        tp_mac(y1, w[7], p1);
+        v[r].x = on ? y0.x : a0.x;
+        v[r].y = on ? y0.y : a0.y;
+        v[r | b].x = on ? y1.x : a1.x;
+        v[r | b].y = on ? y1.y : a1.y;
+    }
+}
+
+template <class T, int KB, bool CHK, int J = 0>
+__device__ __forceinline__ void tr_mix1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
+                                           int src, bool on) {
+    if constexpr (J < KB) {
+        if (in.ja == J) {
+            tr_mix1<T, KB, J, CHK>(v, in, w, src, on);
+        }
+        tr_mix1_at<T, KB, CHK, J + 1>(v, in, w, src, on);
+    }
+}
+
+// KB = k - 6: the number of local bits held in registers.  One wavefront
+// per tile; `lds_bytes` of dynamic LDS only when the pass has lds ops.
+template <class T, int KB>
+__global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, uint64_t ntiles,
+                                                        const TpGeom g, const TrInst<T> *__restrict__ prog,
+                                                        int ninst, const TpOp<T> *__restrict__ lops) {
+    extern __shared__ unsigned char smem_raw[];
+    v2<T> *lds = reinterpret_cast<v2<T> *>(smem_raw);
+    constexpr int R = 1 << KB;
+    const uint64_t t = xcd_block();
+    if (t >= ntiles) {
+        return;
+    }
+    const int lane = threadIdx.x;
+    const uint64_t base = expand(t << g.low, g.outer);
+    // HBM offsets of the lane's local bits 0..5 and of each register's bits
+    uint64_t loff = static_cast<uint64_t>(lane) & ((uint64_t{1} << g.low) - 1);
+    const int nlane_hi = 6 - g.low;
+    for (int j = 0; j < nlane_hi; j++) {
+        if ((lane >> (g.low + j)) & 1) {
+            loff |= uint64_t{1} << g.hb[j];
+        }
+    }
+    v2<T> *const p = sv + (base | loff);
+    uint64_t roff[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            if ((r >> j) & 1) {
+                o |= uint64_t{1} << g.hb[nlane_hi + j];
+            }
+        }
+        roff[r] = o;
+    }
+    v2<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        v[r] = ld_nt<T>(p + roff[r]);
+    }
+    const int k = 6 + KB;
+    for (int pc = 0; pc < ninst; pc++) {
+        const TrInst<T> &in = prog[pc];
+        const int code = in.code;
+        if (code == kTrDiag) {
+            // per-lane phases for each combination of the register-fed index bits
+            const bool tc = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
+            int ct = 0;
+            if (in.tcb[0] >= 0) {
+                ct |= (lane >> in.tcb[0]) & 1;
+            }
+            if (in.tcb[1] >= 0) {
+                ct |= ((lane >> in.tcb[1]) & 1) << 1;
+            }
+            const dcx<T> d0 = in.mat[0], d1 = in.mat[1], d2 = in.mat[2], d3 = in.mat[3];
+            const int nr = in.nr;
+            dcx<T> pt[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                pt[q] = dcx<T>{T(1), T(0)};
+                if (q < (1 << nr)) {
+                    int c = ct;
+                    if (q & 1) {
+                        c |= 1 << in.rcb[0];
+                    }
+                    if (q & 2) {
+                        c |= 1 << in.rcb[1];
+                    }
+                    // two-level select on the index bits (components kept apart)
+                    const bool b0 = c & 1, b1 = c & 2;
+                    const T lr = b0 ? d1.r : d0.r, li = b0 ? d1.i : d0.i;
+                    const T hr = b0 ? d3.r : d2.r, hi = b0 ? d3.i : d2.i;
+                    pt[q].r = tc ? (b1 ? hr : lr) : T(1);
+                    pt[q].i = tc ? (b1 ? hi : li) : T(0);
+                }
+            }
+            if (nr == 0) {
+                tr_diag<T, KB, 0, 0, 0>(v, in, pt);
+            } else if (nr == 1) {
+                tr_diag1_at<T, KB>(v, in, pt);
+            } else {
+                tr_diag2_at<T, KB>(v, in, pt);
+            }
+        } else if (code == kTrParity) {
+            const bool odd = __popc(static_cast<uint32_t>(lane) & in.tmask) & 1;
+            const dcx<T> e0 = in.mat[0], e1 = in.mat[1];
+            const dcx<T> p0{odd ? e1.r : e0.r, odd ? e1.i : e0.i};
+            const dcx<T> p1{odd ? e0.r : e1.r, odd ? e0.i : e1.i};
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const bool rodd = __popc(static_cast<uint32_t>(r) & in.rmask) & 1;
+                v[r] = tp_mul(dcx<T>{rodd ? p1.r : p0.r, rodd ? p1.i : p0.i}, v[r]);
+            }
+        } else if (code == kTrLane) {
+            // partner amplitude from lane ^ lx; every lane takes part in the exchange
+            const bool hi = (static_cast<uint32_t>(lane) & in.lx) != 0;
+            const bool on = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
+            const dcx<T> m0 = in.mat[0], m1 = in.mat[1], m2 = in.mat[2], m3 = in.mat[3];
+            const dcx<T> a{hi ? m3.r : m0.r, hi ? m3.i : m0.i}; // own amplitude
+            const dcx<T> b{hi ? m2.r : m1.r, hi ? m2.i : m1.i}; // partner amplitude
+            const int src = lane ^ static_cast<int>(in.lx);
+            if (in.rmask == 0 && in.tmask == 0) { // no controls
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const v2<T> o = tp_lane_xor<T>(v[r], src);
+                    v[r] = tp_mul2(a, v[r], b, o);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+                        continue;
+                    }
+                    const v2<T> o = tp_lane_xor<T>(v[r], src);
+                    const v2<T> y = tp_mul2(a, v[r], b, o);
+                    v[r].x = on ? y.x : v[r].x; // componentwise: no aggregate select
+                    v[r].y = on ? y.y : v[r].y;
+                }
+            }
+        } else if (code == kTrMix1) {
+            const bool hi = (static_cast<uint32_t>(lane) & in.lx) != 0;
+            const bool on = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
+            dcx<T> w[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const dcx<T> lo_ = in.mat[c], hi_ = in.mat[8 + c];
+                w[c] = dcx<T>{hi ? hi_.r : lo_.r, hi ? hi_.i : lo_.i};
+            }
+            const int src = lane ^ static_cast<int>(in.lx);
+            if (in.rmask != 0) {
+                tr_mix1_at<T, KB, true>(v, in, w, src, on);
+            } else {
+                tr_mix1_at<T, KB, false>(v, in, w, src, on);
+            }
+        } else if (code == kTrMix2) {
+            // two lane member bits: three exchanges per register
+            const int o = ((static_cast<uint32_t>(lane) & in.lx) ? 1 : 0) | ((static_cast<uint32_t>(lane) & in.lx2) ? 2 : 0);
+            const bool on = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
+            dcx<T> w[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                const dcx<T> e0 = in.mat[d], e1 = in.mat[4 + d], e2 = in.mat[8 + d], e3 = in.mat[12 + d];
+                const bool b0 = o & 1, b1 = o & 2;
+                const T lr = b0 ? e1.r : e0.r, li = b0 ? e1.i : e0.i;
+                const T hr = b0 ? e3.r : e2.r, hi = b0 ? e3.i : e2.i;
+                w[d] = dcx<T>{b1 ? hr : lr, b1 ? hi : li};
+            }
+            const int l1 = static_cast<int>(in.lx), l2 = static_cast<int>(in.lx2);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+                    continue;
+                }
+                const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
+                const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
+                const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
+                v2<T> y = tp_mul(w[0], v[r]);
+                tp_mac(y, w[1], s1);
+                tp_mac(y, w[2], s2);
+                tp_mac(y, w[3], s3);
+                v[r].x = on ? y.x : v[r].x;
+                v[r].y = on ? y.y : v[r].y;
+            }
+        } else if (code == kTrLds) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                lds[lane | (r << 6)] = v[r];
+            }
+            __syncthreads();
+            const TpOp<T> &op = lops[in.lds_op];
+            if (op.kind == 2) {
+                tp_parity<T>(lds, op, k, lane);
+            } else if (op.M == 2) {
+                tp_group<T, 2>(lds, op, k, lane);
+            } else if (op.M == 4) {
+                tp_group<T, 4>(lds, op, k, lane);
+            } else if (op.M == 1) {
+                tp_group<T, 1>(lds, op, k, lane);
+            } else {
+                tp_group<T, 8>(lds, op, k, lane);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                v[r] = lds[lane | (r << 6)];
+            }
+            __syncthreads();
+        } else if ((static_cast<uint32_t>(lane) & in.tmask) == in.tval) {
+            const bool chk = in.rmask != 0;
+            if (code == kTrP1) {
+                if (chk) {
+                    tr_p1_at<T, KB, true>(v, in);
+                } else {
+                    tr_p1_at<T, KB, false>(v, in);
+                }
+            } else if (code == kTrP2) {
+                if (chk) {
+                    tr_pair_at<T, KB, false, true>(v, in);
+                } else {
+                    tr_pair_at<T, KB, false, false>(v, in);
+                }
+            } else if (chk) {
+                tr_pair_at<T, KB, true, true>(v, in);
+            } else {
+                tr_pair_at<T, KB, true, false>(v, in);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        st_nt<T>(p + roff[r], v[r]);
+    }
+}
+
+} // namespace plamd

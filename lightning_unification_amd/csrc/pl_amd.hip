@@ -27,6 +27,7 @@
 #include "kernels_mcmc.hpp"
 #include "fuse.hpp"
 #include "lower.hpp"
+#include "tile.hpp"
 
 namespace plamd {
 
@@ -636,6 +637,133 @@ template <class T> struct Engine {
             *keep = std::move(blocks);
         }
         return nb;
+    }
+
+    // ---- tile passes: many gates per read and write of the state --------------
+    // measured on MI355X (the 50-layer random circuit): complex128 at 30 qubits
+    // is fastest with 2^10-amplitude tiles and 3 low bits, complex64 at 33
+    // qubits with 2^11 and 4 (register pressure bounds larger tiles)
+    static constexpr int kTileDefaultBits = std::is_same_v<T, double> ? 10 : 11;
+    static constexpr int kTileDefaultLow = std::is_same_v<T, double> ? 3 : 4;
+
+    // one wavefront per tile; LDS only for passes with lds ops
+    template <int KB>
+    void launch_tile_pass(V *sv, uint64_t ntiles, const TpGeom &g, const TrInst<T> *prog, int ninst,
+                          const TpOp<T> *lops, bool uses_lds) {
+        const size_t lds = uses_lds ? (size_t{1} << (6 + KB)) * sizeof(V) : 0;
+        hipLaunchKernelGGL((k_tile_pass<T, KB>), dim3(static_cast<unsigned>(ntiles)), dim3(kTpLanes), lds,
+                           h->stream, sv, ntiles, g, prog, ninst, lops);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Applies a tape as tile passes (tile.hpp); returns the number of launches.
+    // k = 0 / low < 0 pick the measured defaults of the precision; with k = 0
+    // a state narrower than the smallest tile runs gate by gate.
+    int apply_tiled(V *sv, const TapeView &t, int k, int low) {
+        std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
+        for (int i = 0; i < t.nops; i++) {
+            woff[i + 1] = woff[i] + t.nwires[i];
+            poff[i + 1] = poff[i] + t.nparams[i];
+        }
+        if (k == 0) {
+            if (n() < kTpMinBits) {
+                int launches = 0;
+                for (int i = 0; i < t.nops; i++) {
+                    if (t.ids[i] != -1) {
+                        gate(sv, t.ids[i], t.wires + woff[i], t.nwires[i], t.inverse[i] != 0, t.params + poff[i],
+                             t.nparams[i]);
+                        launches++;
+                    }
+                }
+                HIP_OK(hipStreamSynchronize(h->stream));
+                return launches;
+            }
+            k = std::min(n(), kTileDefaultBits);
+        }
+        if (low < 0) {
+            low = std::min(kTileDefaultLow, k - 6);
+        }
+        const auto passes = tile_schedule(n(), t, k, low);
+        int dev_id = 0, lds_max = 0;
+        HIP_OK(hipGetDevice(&dev_id));
+        HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id));
+        if ((size_t{1} << k) * sizeof(V) > static_cast<size_t>(lds_max)) {
+            throw InvalidArg("a tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS of a workgroup");
+        }
+        // every pass's program in one device buffer, the LDS-applied ops after them
+        std::vector<TrInst<T>> prog;
+        std::vector<TpOp<T>> lops;
+        std::vector<std::pair<size_t, int>> range(passes.size(), {0, 0});
+        std::vector<char> uses_lds(passes.size(), 0);
+        for (size_t p = 0; p < passes.size(); p++) {
+            if (!passes[p].tiled) {
+                continue;
+            }
+            range[p].first = prog.size();
+            const size_t nl = lops.size();
+            tile_program<T>(n(), t, passes[p], low, woff, poff, prog, lops);
+            range[p].second = static_cast<int>(prog.size() - range[p].first);
+            uses_lds[p] = lops.size() > nl;
+        }
+        const size_t pbytes = (prog.size() * sizeof(TrInst<T>) + 255) & ~size_t{255};
+        const size_t bytes = pbytes + lops.size() * sizeof(TpOp<T>);
+        unsigned char *dev = nullptr;
+        if (bytes) {
+            HIP_OK(hipMalloc(&dev, bytes));
+        }
+        struct Free {
+            void *p;
+            hipStream_t s;
+            ~Free() {
+                if (p) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipFree(p);
+                }
+            }
+        } guard{dev, h->stream};
+        if (!prog.empty()) {
+            HIP_OK(hipMemcpyAsync(dev, prog.data(), prog.size() * sizeof(TrInst<T>), hipMemcpyHostToDevice,
+                                  h->stream));
+        }
+        if (!lops.empty()) {
+            HIP_OK(hipMemcpyAsync(dev + pbytes, lops.data(), lops.size() * sizeof(TpOp<T>),
+                                  hipMemcpyHostToDevice, h->stream));
+        }
+        const auto *dprog = reinterpret_cast<const TrInst<T> *>(dev);
+        const auto *dlops = reinterpret_cast<const TpOp<T> *>(dev + pbytes);
+        const uint64_t ntiles = uint64_t{1} << (n() - k);
+        for (size_t p = 0; p < passes.size(); p++) {
+            const auto &ps = passes[p];
+            if (!ps.tiled) {
+                const int o = ps.ops[0];
+                gate(sv, t.ids[o], t.wires + woff[o], t.nwires[o], t.inverse[o] != 0, t.params + poff[o],
+                     t.nparams[o]);
+                continue;
+            }
+            if (range[p].second == 0) {
+                continue;
+            }
+            TpGeom g{};
+            g.k = k;
+            g.low = low;
+            g.outer.nb = static_cast<int>(ps.hb.size());
+            g.outer.fixed = 0;
+            for (size_t j = 0; j < ps.hb.size(); j++) {
+                g.hb[j] = ps.hb[j];
+                g.outer.pos[j] = ps.hb[j];
+            }
+            const TrInst<T> *pp = dprog + range[p].first;
+            const int ni = range[p].second;
+            const bool ul = uses_lds[p] != 0;
+            switch (k - 6) {
+            case 3: launch_tile_pass<3>(sv, ntiles, g, pp, ni, dlops, ul); break;
+            case 4: launch_tile_pass<4>(sv, ntiles, g, pp, ni, dlops, ul); break;
+            case 5: launch_tile_pass<5>(sv, ntiles, g, pp, ni, dlops, ul); break;
+            default: launch_tile_pass<6>(sv, ntiles, g, pp, ni, dlops, ul); break;
+            }
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return static_cast<int>(passes.size());
     }
 
     // ---- reductions ----------------------------------------------------------
@@ -1668,6 +1796,54 @@ int pl_sv_apply_ops_fused(pl_sv_t h, int nops, const int *ids, const int *nwires
         const int nb = E.apply_fused(E.psi(), t, max_wires, flags, nullptr);
         if (nblocks) {
             *nblocks = nb;
+        }
+    });
+}
+
+int pl_sv_apply_ops_tiled(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
+                          const uint8_t *inverse, const int *nparams, const double *params,
+                          int tile_bits, int low_bits, int *npasses) {
+    return dispatch(h, [&](auto &E) {
+        if (nops < 0 || (nops > 0 && (!ids || !nwires || !wires || !inverse || !nparams))) {
+            throw InvalidArg("null tape argument");
+        }
+        const TapeView t{nops, ids, nwires, wires, inverse, nparams, params};
+        const int np = E.apply_tiled(E.psi(), t, tile_bits, low_bits);
+        if (npasses) {
+            *npasses = np;
+        }
+    });
+}
+
+int pl_tile_plan(int num_qubits, int nops, const int *ids, const int *nwires, const int *wires,
+                 const uint8_t *inverse, const int *nparams, const double *params, int tile_bits,
+                 int low_bits, int capacity, int *npasses, int *pass_nops, int *pass_tiled,
+                 int *pass_bits, int *order) {
+    return guarded([&] {
+        if (num_qubits < 1 || num_qubits > 62 || nops < 0 || !npasses) {
+            throw InvalidArg("invalid argument");
+        }
+        if (nops > 0 && (!ids || !nwires || !wires || !inverse || !nparams)) {
+            throw InvalidArg("null tape argument");
+        }
+        const TapeView t{nops, ids, nwires, wires, inverse, nparams, params};
+        const auto passes = tile_schedule(num_qubits, t, tile_bits, low_bits);
+        *npasses = static_cast<int>(passes.size());
+        if (*npasses > capacity) {
+            throw InvalidArg("capacity too small for " + std::to_string(*npasses) + " passes");
+        }
+        int at = 0;
+        for (size_t p = 0; p < passes.size(); p++) {
+            const auto &ps = passes[p];
+            if (pass_nops) pass_nops[p] = static_cast<int>(ps.ops.size());
+            if (pass_tiled) pass_tiled[p] = ps.tiled ? 1 : 0;
+            for (int j = 0; pass_bits && j < 16; j++) {
+                pass_bits[16 * p + j] = j < static_cast<int>(ps.hb.size()) ? ps.hb[j] : -1;
+            }
+            for (int i : ps.ops) {
+                if (order) order[at] = i;
+                at++;
+            }
         }
     });
 }

@@ -1,0 +1,367 @@
+// Tile-pass scheduling (host only): groups the operations of a tape into
+// passes of k_tile_pass.
+//
+// A pass owns the `low` lowest bit positions and up to k - low higher ones.
+// Operations are taken in tape order; an operation joins the open pass when
+//   - it lowers to a group op of at most kTpMaxM amplitudes and kTpMaxPins
+//     pins (or to a parity phase),
+//   - its high bits fit in the pass's remaining budget, and
+//   - it shares no bit with any earlier operation left out of the pass.
+// The last rule makes the reordering exact: an operation only moves ahead of
+// operations on disjoint bits, with which it commutes.  Operations left out
+// keep their relative order and are scheduled into later passes.  An
+// operation that can never be tiled (too many amplitudes per group) becomes a
+// pass of its own, applied by the per-gate kernels.  The high bits are padded
+// with the lowest unused positions, which lengthens the contiguous runs.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <vector>
+
+#include "fuse.hpp"
+#include "kernels_tile.hpp"
+#include "lower.hpp"
+
+namespace plamd {
+
+constexpr size_t kTileLookahead = 4096; // operations scanned per pass
+
+struct TilePass {
+    bool tiled = true;        // false: ops[0] applied by the per-gate kernels
+    std::vector<int> hb;      // bit positions of local bits low.. (ascending)
+    std::vector<int> ops;     // tape op indices in application order
+};
+
+// Does the operation lower to something a tile pass can apply?
+inline bool tile_op_ok(int id, int nw, bool inv, const double *prm, int np) {
+    if (nw < 1 || nw > kTpMaxPins) {
+        return false;
+    }
+    std::vector<int> lw(nw);
+    for (int a = 0; a < nw; a++) {
+        lw[a] = a;
+    }
+    const auto g = lower_gate<double>(nw, id, lw.data(), nw, inv, prm, np);
+    if (g.kind == GroupOp<double>::Noop || g.kind == GroupOp<double>::Parity) {
+        return true;
+    }
+    return (g.M == 1 || g.M == 2 || g.M == 4 || g.M == 8) && g.nb <= kTpMaxPins;
+}
+
+inline void tile_check_args(int n, int k, int low) {
+    if (k < kTpMinBits || k > kTpMaxBits) {
+        throw InvalidArg("tile bits must be in [9, 12]");
+    }
+    if (low < 0 || low > kTpMaxLow) {
+        throw InvalidArg("low bits must be in [0, 6]");
+    }
+    if (n < k) {
+        throw InvalidArg("tile bits exceed the qubit count");
+    }
+}
+
+inline std::vector<TilePass> tile_schedule(int n, const TapeView &t, int k, int low) {
+    tile_check_args(n, k, low);
+    const int budget = k - low;
+    std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
+    for (int i = 0; i < t.nops; i++) {
+        if (t.nwires[i] < 0 || t.nparams[i] < 0) {
+            throw InvalidArg("negative wire or parameter count");
+        }
+        woff[i + 1] = woff[i] + t.nwires[i];
+        poff[i + 1] = poff[i] + t.nparams[i];
+    }
+    const uint64_t low_mask = (uint64_t{1} << low) - 1;
+    const uint64_t all = n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1;
+    std::vector<uint64_t> bits(t.nops, 0);
+    std::vector<char> ok(t.nops, 0);
+    std::vector<int> pending;
+    for (int i = 0; i < t.nops; i++) {
+        if (t.ids[i] == -1) {
+            continue; // state preparation: skipped, as by applyOperations
+        }
+        for (int a = 0; a < t.nwires[i]; a++) {
+            const int w = t.wires[woff[i] + a];
+            if (w < 0 || w >= n) {
+                throw InvalidArg("wire index out of range");
+            }
+            bits[i] |= uint64_t{1} << (n - 1 - w);
+        }
+        ok[i] = tile_op_ok(t.ids[i], t.nwires[i], t.inverse[i] != 0, t.params + poff[i], t.nparams[i]) &&
+                __builtin_popcountll(bits[i] & ~low_mask) <= budget;
+        pending.push_back(i);
+    }
+    std::vector<TilePass> out;
+    std::vector<int> rest;
+    while (!pending.empty()) {
+        TilePass p;
+        uint64_t hmask = 0, blocked = 0;
+        rest.clear();
+        size_t j = 0;
+        for (; j < pending.size() && j < kTileLookahead; j++) {
+            const int i = pending[j];
+            const uint64_t hi = bits[i] & ~low_mask;
+            if (ok[i] && !(bits[i] & blocked) && __builtin_popcountll(hmask | hi) <= budget) {
+                hmask |= hi;
+                p.ops.push_back(i);
+            } else {
+                blocked |= bits[i];
+                rest.push_back(i);
+                if (blocked == all) {
+                    j++;
+                    break;
+                }
+            }
+        }
+        rest.insert(rest.end(), pending.begin() + static_cast<long>(j), pending.end());
+        if (p.ops.empty()) { // the first pending op cannot be tiled
+            p.tiled = false;
+            p.ops.push_back(pending[0]);
+            rest.erase(rest.begin());
+        } else {
+            for (int b = low; b < n && __builtin_popcountll(hmask) < budget; b++) {
+                hmask |= uint64_t{1} << b;
+            }
+            for (int b = low; b < n; b++) {
+                if ((hmask >> b) & 1U) {
+                    p.hb.push_back(b);
+                }
+            }
+        }
+        out.push_back(std::move(p));
+        pending.swap(rest);
+    }
+    return out;
+}
+
+// Operation i of the tape lowered onto the k-qubit local system of a pass.
+template <class T>
+GroupOp<T> tile_group(int n, const TapeView &t, int i, int woff, int poff, int low, const std::vector<int> &hb) {
+    const int k = low + static_cast<int>(hb.size());
+    std::vector<int> lw(t.nwires[i]);
+    for (int a = 0; a < t.nwires[i]; a++) {
+        const int bit = n - 1 - t.wires[woff + a];
+        int m = bit;
+        if (bit >= low) {
+            const auto it = std::find(hb.begin(), hb.end(), bit);
+            if (it == hb.end()) {
+                throw InvalidArg("operation outside its tile");
+            }
+            m = low + static_cast<int>(it - hb.begin());
+        }
+        lw[a] = k - 1 - m;
+    }
+    std::vector<T> prm(t.nparams[i]);
+    for (int a = 0; a < t.nparams[i]; a++) {
+        prm[a] = static_cast<T>(t.params[poff + a]);
+    }
+    return lower_gate<T>(k, t.ids[i], lw.data(), t.nwires[i], t.inverse[i] != 0, prm.data(), t.nparams[i]);
+}
+
+template <class T> TpOp<T> tile_lds_op(const GroupOp<T> &g) {
+    TpOp<T> op{};
+    if (g.kind == GroupOp<T>::Parity) {
+        op.kind = 2;
+        op.M = 1;
+        op.mask = static_cast<uint32_t>(g.parity_mask);
+        op.mat[0] = {g.parity_phase[0].real(), g.parity_phase[0].imag()};
+        op.mat[1] = {g.parity_phase[1].real(), g.parity_phase[1].imag()};
+        return op;
+    }
+    if (!(g.M == 1 || g.M == 2 || g.M == 4 || g.M == 8) || g.nb > kTpMaxPins) {
+        throw InvalidArg("operation too wide for a tile pass");
+    }
+    op.kind = g.diag ? 1 : 0;
+    op.M = g.M;
+    op.nb = g.nb;
+    for (int a = 0; a < g.nb; a++) {
+        op.pos[a] = g.pos[a];
+    }
+    op.fixed = static_cast<uint32_t>(g.fixed);
+    for (int a = 0; a < g.M; a++) {
+        op.off[a] = static_cast<uint32_t>(g.off[a]);
+    }
+    for (int a = 0; a < g.M * g.M; a++) {
+        op.mat[a] = {g.mat[a].real(), g.mat[a].imag()};
+    }
+    return op;
+}
+
+// Compiles one tiled pass into a wavefront-tile program (kernels_tile.hpp):
+// lane bits are local bits 0..5, register bit j is local bit 6 + j.
+template <class T>
+void tile_program(int n, const TapeView &t, const TilePass &ps, int low, const std::vector<int> &woff,
+                  const std::vector<int> &poff, std::vector<TrInst<T>> &prog, std::vector<TpOp<T>> &lops) {
+    const int k = low + static_cast<int>(ps.hb.size());
+    const uint32_t lane_bits = 63U, reg_bits = ((1U << k) - 1) & ~lane_bits;
+    auto cx = [](const std::complex<T> &z) { return dcx<T>{z.real(), z.imag()}; };
+    for (int i : ps.ops) {
+        const auto g = tile_group<T>(n, t, i, woff[i], poff[i], low, ps.hb);
+        if (g.kind == GroupOp<T>::Noop) {
+            continue;
+        }
+        TrInst<T> in{};
+        if (g.kind == GroupOp<T>::Parity) { // parity of the (lane, register) parts of the mask
+            const uint32_t pm = static_cast<uint32_t>(g.parity_mask);
+            in.code = kTrParity;
+            in.tmask = pm & lane_bits;
+            in.rmask = pm >> 6;
+            in.mat[0] = cx(g.parity_phase[0]);
+            in.mat[1] = cx(g.parity_phase[1]);
+            prog.push_back(in);
+            continue;
+        }
+        uint32_t pins = 0, xo = 0;
+        for (int b = 0; b < g.nb; b++) {
+            pins |= 1U << g.pos[b];
+        }
+        for (int b = 0; b < g.M; b++) {
+            xo |= static_cast<uint32_t>(g.off[b]);
+        }
+        const uint32_t fx = static_cast<uint32_t>(g.fixed);
+        const bool clean = (fx & ~pins) == 0 && (fx & xo) == 0 && (xo & ~pins) == 0;
+        // condition on the non-member pins, split into lane and register bits
+        const uint32_t cmask = pins & ~xo, cval = fx & cmask;
+        in.tmask = cmask & lane_bits;
+        in.tval = cval & lane_bits;
+        in.rmask = cmask >> 6;
+        in.rval = cval >> 6;
+        const int nx = __builtin_popcount(xo);
+        if (clean && (g.M == 1 || g.diag) && g.M <= 4 && nx <= 2) {
+            in.code = kTrDiag;
+            // phase index c: bit i = member bit mb[i]; mat[c] = the phase there (1: no member)
+            int mb[2] = {-1, -1}, nm = 0;
+            for (int b = 0; b < k; b++) {
+                if ((xo >> b) & 1U) {
+                    mb[nm++] = b;
+                }
+            }
+            for (int c = 0; c < 4; c++) {
+                uint32_t x = 0;
+                for (int a = 0; a < nm; a++) {
+                    if ((c >> a) & 1) {
+                        x |= 1U << mb[a];
+                    }
+                }
+                in.mat[c] = dcx<T>{T(1), T(0)};
+                if (c >= (1 << nm)) {
+                    continue;
+                }
+                for (int j = 0; j < g.M; j++) {
+                    if (static_cast<uint32_t>(g.off[j]) == x) {
+                        in.mat[c] = cx(g.mat[j * g.M + j]);
+                    }
+                }
+            }
+            // member bits held in registers feed slots ordered by register bit
+            in.tcb[0] = in.tcb[1] = -1;
+            in.nr = 0;
+            for (int a = 0; a < nm; a++) {
+                if (mb[a] >= 6) {
+                    (in.nr == 0 ? in.ja : in.jb) = mb[a] - 6;
+                    in.rcb[in.nr++] = a;
+                } else {
+                    in.tcb[a] = mb[a];
+                }
+            }
+            prog.push_back(in);
+            continue;
+        }
+        const bool p1 = clean && g.M == 2 && g.off[0] == 0 && nx == 1;
+        const bool p2 = clean && g.M == 2 && nx == 2 && __builtin_popcountll(g.off[0]) == 1 &&
+                        __builtin_popcountll(g.off[1]) == 1;
+        const bool p4 = clean && g.M == 4 && g.off[0] == 0 && nx == 2 && __builtin_popcountll(g.off[1]) == 1 &&
+                        __builtin_popcountll(g.off[2]) == 1 && g.off[3] == (g.off[1] | g.off[2]);
+        if ((p1 || p2 || p4) && (xo & ~reg_bits) == 0) { // members in registers
+            std::vector<std::complex<T>> m = g.mat;
+            if (p1) {
+                in.code = kTrP1;
+                in.ja = __builtin_ctz(xo) - 6;
+            } else {
+                in.code = p2 ? kTrP2 : kTrP4;
+                const int ra = __builtin_ctzll(g.off[p2 ? 0 : 1]) - 6;
+                const int rb = __builtin_ctzll(g.off[p2 ? 1 : 2]) - 6;
+                in.ja = std::min(ra, rb);
+                in.jb = std::max(ra, rb);
+                if (ra > rb) { // members listed in the other order: permute the matrix
+                    const int M = g.M;
+                    const std::vector<int> perm = M == 2 ? std::vector<int>{1, 0} : std::vector<int>{0, 2, 1, 3};
+                    for (int r = 0; r < M; r++) {
+                        for (int c = 0; c < M; c++) {
+                            m[r * M + c] = g.mat[perm[r] * M + perm[c]];
+                        }
+                    }
+                }
+            }
+            for (int b = 0; b < g.M * g.M; b++) {
+                in.mat[b] = cx(m[b]);
+            }
+            prog.push_back(in);
+            continue;
+        }
+        if (p1) { // 2x2 on a lane bit
+            in.code = kTrLane;
+            in.lx = xo;
+            for (int b = 0; b < 4; b++) {
+                in.mat[b] = cx(g.mat[b]);
+            }
+            prog.push_back(in);
+            continue;
+        }
+        if (clean && g.kind == GroupOp<T>::Group && g.M <= 4 && nx == 2) {
+            // two member bits, at least one a lane bit: the op as a 4x4 matrix U
+            // over the patterns u of (mb[0], mb[1]) (identity where no member)
+            int mb[2], nm = 0;
+            for (int b = 0; b < k; b++) {
+                if ((xo >> b) & 1U) {
+                    mb[nm++] = b;
+                }
+            }
+            auto pat = [&](uint64_t off) { return static_cast<int>(((off >> mb[0]) & 1U) | (((off >> mb[1]) & 1U) << 1)); };
+            std::complex<T> U[4][4] = {};
+            for (int a = 0; a < 4; a++) {
+                U[a][a] = 1;
+            }
+            for (int a = 0; a < g.M; a++) {
+                for (int c = 0; c < g.M; c++) {
+                    U[pat(g.off[a])][pat(g.off[c])] = g.mat[a * g.M + c];
+                }
+            }
+            if (mb[1] >= 6) { // mb[0] lane bit, mb[1] register bit
+                const int iq = 0, ij = 1;
+                in.code = kTrMix1;
+                in.lx = 1U << mb[iq];
+                in.ja = mb[ij] - 6;
+                auto u = [&](int qb, int jb) { return (qb << iq) | (jb << ij); };
+                for (int oq = 0; oq < 2; oq++) {
+                    for (int jb = 0; jb < 2; jb++) {
+                        const int row = u(oq, jb);
+                        const int cols[4] = {u(oq, 0), u(oq, 1), u(1 - oq, 0), u(1 - oq, 1)};
+                        for (int c = 0; c < 4; c++) {
+                            in.mat[8 * oq + 4 * jb + c] = cx(U[row][cols[c]]);
+                        }
+                    }
+                }
+            } else { // both lane bits
+                in.code = kTrMix2;
+                in.lx = 1U << mb[0];
+                in.lx2 = 1U << mb[1];
+                for (int o = 0; o < 4; o++) {
+                    for (int d = 0; d < 4; d++) {
+                        in.mat[4 * o + d] = cx(U[o][o ^ d]);
+                    }
+                }
+            }
+            prog.push_back(in);
+            continue;
+        }
+        in = TrInst<T>{};
+        in.code = kTrLds;
+        in.lds_op = static_cast<int>(lops.size());
+        lops.push_back(tile_lds_op(g));
+        prog.push_back(in);
+    }
+}
+
+} // namespace plamd

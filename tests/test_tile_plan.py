@@ -1,0 +1,102 @@
+"""Tile-pass plan (pl_tile_plan, host only).  Applying the operations in the
+plan's order must reproduce the oracle's run of the tape in tape order;
+every tiled pass holds only operations whose bits lie in its tile; an
+operation moves ahead of an earlier one only when their wires are disjoint
+(so the two commute); every operation appears exactly once."""
+import numpy as np
+import pytest
+
+from conftest import random_state, rel_err
+from lightning_unification_amd.circuits import Tape, random_layers
+from lightning_unification_amd.gates import GATE_ID, GATE_NAMES, GATE_NPARAMS, GATE_WIRES
+
+
+def zoo_tape(n, seed, count=120):
+    """Random layers, then `count` operations drawn over every gate kind with
+    random distinct wires, parameters and inverse flags; one state
+    preparation op (skipped by applyOperations) in the middle."""
+    rs = np.random.default_rng(seed)
+    t = random_layers(n, 3, seed=seed)
+    for c in range(count):
+        gi = int(rs.integers(len(GATE_NAMES)))
+        k = GATE_WIRES[gi] if GATE_WIRES[gi] > 0 else int(rs.integers(1, min(n, 7) + 1))
+        wires = [int(w) for w in rs.choice(n, k, replace=False)]
+        t.add(GATE_NAMES[gi], wires, list(rs.uniform(-7, 7, GATE_NPARAMS[gi])), bool(rs.integers(2)))
+        if c == count // 2:
+            t.ops.append(("BasisState", [0], False, []))
+    return t
+
+
+def without_prep(tape):
+    t = Tape()
+    t.ops = [op for op in tape.ops if op[0] in GATE_ID]
+    return t
+
+
+def op_bits(n, op):
+    m = 0
+    for w in op[1]:
+        m |= 1 << (n - 1 - w)
+    return m
+
+
+@pytest.mark.parametrize("n,k,low", [(9, 9, 3), (10, 9, 1), (11, 10, 4), (12, 12, 6), (12, 11, 0),
+                                     (13, 12, 1)])
+def test_plan_reproduces_tape(oracle, n, k, low):
+    from lightning_unification_amd import capi
+    tape = zoo_tape(n, 100 + n + k + low)
+    plan = capi.tile_plan(n, tape, k, low)
+    prep = [i for i, op in enumerate(tape.ops) if op[0] not in GATE_ID]
+    order = [i for _, _, ops in plan for i in ops]
+    assert sorted(order) == [i for i in range(len(tape.ops)) if i not in prep]
+    low_mask = (1 << low) - 1
+    for tiled, hb, ops in plan:
+        if not tiled:
+            assert len(ops) == 1
+            continue
+        assert len(hb) == k - low and hb == sorted(hb) and all(low <= b < n for b in hb)
+        allowed = low_mask | sum(1 << b for b in hb)
+        for i in ops:
+            assert op_bits(n, tape.ops[i]) & ~allowed == 0, (tape.ops[i], hb)
+    # an operation only overtakes earlier operations on disjoint wires
+    pos = {i: p for p, i in enumerate(order)}
+    for a in order:
+        for b in order:
+            if a < b and pos[a] > pos[b]:
+                assert op_bits(n, tape.ops[a]) & op_bits(n, tape.ops[b]) == 0
+    # replaying in plan order reproduces the tape
+    psi = random_state(n, 3)
+    exp = oracle.apply_tape(psi.copy(), without_prep(tape))
+    re = Tape()
+    re.ops = [tape.ops[i] for i in order]
+    got = oracle.apply_tape(psi.copy(), re)
+    assert rel_err(got, exp) < 1e-12
+
+
+def test_plan_counts_on_the_30_qubit_circuit():
+    from lightning_unification_amd import capi
+    tape = random_layers(30, 50, seed=1002)
+    plan = capi.tile_plan(30, tape, 12, 6)
+    assert all(t for t, _, _ in plan)
+    assert sum(len(ops) for _, _, ops in plan) == len(tape.ops)
+    # the greedy schedule with commutation look-ahead packs ~10 gates per pass
+    assert len(plan) <= len(tape.ops) // 8
+
+
+def test_wide_operations_are_not_tiled():
+    from lightning_unification_amd import capi
+    n = 12
+    t = Tape()
+    t.add("Hadamard", [0])
+    t.add("MultiRZ", list(range(12)), [0.4])  # 12 high bits: wider than any tile
+    t.add("RX", [3], [0.2])
+    plan = capi.tile_plan(n, t, 9, 3)
+    assert [p[0] for p in plan] == [True, False, True]
+    assert [p[2] for p in plan] == [[0], [1], [2]]
+
+
+@pytest.mark.parametrize("n,k,low", [(12, 8, 3), (12, 14, 3), (12, 12, 9), (12, 13, 0), (10, 11, 3)])
+def test_plan_argument_errors(n, k, low):
+    from lightning_unification_amd import capi
+    with pytest.raises(Exception):
+        capi.tile_plan(n, random_layers(n, 1, seed=1), k, low)
