@@ -7,9 +7,10 @@ HBM.  value = algorithmic bytes moved by the gates / device time, where a gate
 that can change a fraction f of the amplitudes moves 2 * f * 2^n * 16 bytes.
 Every launch is bracketed by HIP events on the engine's stream (the stream
 the kernels run on).  Also reported: the 100-term Hamiltonian expectation and
-10^6 inverse-CDF samples on the final state, and a CPU baseline (the
-reference's Lightning-Qubit kernels compiled from source, or the oracle
-restatement) on a bounded sample of the same gates.
+10^6 inverse-CDF samples on the final state, the same circuit through the
+tile-pass executor (many gates per read and write of the state), and a CPU
+baseline (the reference's Lightning-Qubit kernels compiled from source, or the
+oracle restatement) on a bounded sample of the same gates.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N]
 """
@@ -254,6 +255,39 @@ def bench_c1():
     return out
 
 
+def bench_tiled(sv_ref, tape, n, reps):
+    """The same circuit through the tile-pass executor (pl_sv_apply_ops_tiled,
+    default tile shape): `reps` applications to a fresh |0>, each timed with
+    HIP events on the engine's stream; then ||psi_tiled - psi_per_gate||_2
+    against sv_ref, which holds the per-gate path after as many applications.
+    hbm_GBs counts one read and one write of the state per pass (the bytes a
+    pass must move); gate_equivalent_GBs the per-gate algorithmic bytes of the
+    circuit over the same time."""
+    from lightning_unification_amd import capi
+    S = (1 << n) * 16
+    gate_bytes = sum(2 * touched_fraction(op[0]) * S for op in tape.ops)
+    sv = capi.DeviceStateVector(n, np.complex128)
+    times, npass = [], 0
+    for r in range(reps):
+        sv.synchronize()
+        sv.event_record(0)
+        npass = sv.apply_tape_tiled(tape)
+        sv.event_record(1)
+        sv.synchronize()
+        times.append(sv.event_elapsed(0, 1))
+    dist = sv.distance(sv_ref)
+    sv.close()
+    ms = min(times)
+    out = {"passes": npass, "gates": len(tape), "ms_per_circuit": round(ms, 2),
+           "ms_mean": round(sum(times) / len(times), 2),
+           "hbm_GBs": round(npass * 2 * S / (ms / 1e3) / 1e9, 1),
+           "hbm_frac": round(npass * 2 * S / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+           "gate_equivalent_GBs": round(gate_bytes / (ms / 1e3) / 1e9, 1),
+           "l2_vs_per_gate": dist}
+    log(f"tiled: {npass} passes, {ms:.1f} ms per circuit, |dpsi| {dist:.2e}")
+    return out
+
+
 def bench_precision(n_big=33, n_ref=30):
     """Precision form: the same seeded 50-layer gate mix, consecutive gates
     fused into 5-qubit 32x32 blocks applied to a complex64 state on FP32
@@ -277,10 +311,17 @@ def bench_precision(n_big=33, n_ref=30):
     s64.apply_tape_fused(tape, 5, renorm=False)
     drift = abs(s64.norm2() - 1.0)
     s64.close()
+    s64 = capi.DeviceStateVector(n_ref, np.complex64)
+    s64.apply_tape_tiled(tape)
+    et = s64.expval(ham)
+    dist_t = s64.distance(s128)
+    drift_t = abs(s64.norm2() - 1.0)
+    s64.close()
     s128.close()
     out["error_vs_c128"] = {"qubits": n_ref, "blocks": nb, "abs_dH": abs(e64 - e128),
                             "H_c128": e128, "state_l2": dist,
-                            "norm_drift_without_renorm": drift}
+                            "norm_drift_without_renorm": drift,
+                            "tiled_c64": {"abs_dH": abs(et - e128), "state_l2": dist_t, "norm_drift": drift_t}}
     log(f"precision n={n_ref}: |dH| {abs(e64 - e128):.2e}, |dpsi| {dist:.2e}, drift {drift:.2e}")
 
     # throughput at n_big
@@ -302,7 +343,18 @@ def bench_precision(n_big=33, n_ref=30):
     e = sv.expval(ham)
     t_h = time.perf_counter() - t0
     norm = sv.norm2()
+    sv.reset()
+    sv.synchronize()
+    sv.timer_start()
+    npass = sv.apply_tape_tiled(tape)
+    ms_t = sv.timer_stop()
+    norm_t = sv.norm2()
     sv.close()
+    out["c64_tiled"] = {"qubits": n_big, "gates": len(tape), "passes": npass, "ms": round(ms_t, 1),
+                        "hbm_GBs": round(npass * 2 * S / (ms_t / 1e3) / 1e9, 1),
+                        "gate_equivalent_GBs": round(gate_bytes / (ms_t / 1e3) / 1e9, 1),
+                        "norm": norm_t}
+    log(f"precision n={n_big} tiled: {npass} passes in {ms_t:.0f} ms")
     out["c64_fused"] = {"qubits": n_big, "gates": len(tape), "blocks": nb, "ms": round(ms, 1),
                         "block_GBs": round(block_bytes / (ms / 1e3) / 1e9, 1),
                         "gate_equivalent_GBs": round(gate_bytes / (ms / 1e3) / 1e9, 1),
@@ -378,6 +430,7 @@ def main():
     ap.add_argument("--surrogate", type=int, default=1 << 20,
                     help="training tuples of the learned-surrogate form (0 skips it)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 20-qubit adjoint measurement")
+    ap.add_argument("--no-tiled", action="store_true", help="skip the tile-pass measurement")
     ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate", "c1"],
                     default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
@@ -474,6 +527,7 @@ def main():
     idx = sv.sample(uniforms)
     t_sample = time.perf_counter() - t0
     log(f"<H> = {e:.12f} ({t_expval*1e3:.1f} ms), {len(idx)} samples ({t_sample*1e3:.1f} ms)")
+    tiled = None if args.no_tiled else bench_tiled(sv, tape, n, args.warmup + args.steps)
     sv.close()
 
     c1 = None if args.no_c1 else bench_c1()
@@ -508,6 +562,7 @@ def main():
         "per_gate_GBs": per_gate_gbs,
         "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},
         "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
+        "tiled_circuit": tiled,
         "c1_adjoint": c1,
         "batched": batched,
         "precision": precision,
