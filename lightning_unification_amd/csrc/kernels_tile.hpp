@@ -555,9 +555,26 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
             }
         }
     }
+    // store offsets recomputed from the kernel arguments: the empty asm keeps
+    // the compiler from holding the load-time offsets in registers through
+    // the whole program
+    uint64_t soff[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        st_nt<T>(p + roff[r], v[r]);
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            int hbj = g.hb[nlane_hi + j];
+            asm volatile("" : "+s"(hbj));
+            if ((r >> j) & 1) {
+                o |= uint64_t{1} << hbj;
+            }
+        }
+        soff[r] = o;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        st_nt<T>(p + soff[r], v[r]);
     }
 }
 
