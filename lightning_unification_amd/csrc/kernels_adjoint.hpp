@@ -26,6 +26,9 @@ template <class T, int M> struct AdjOp {
 };
 
 constexpr int kAdjMaxObs = 16; // observables per launch
+// bytes of states one adjoint sweep keeps in flight when the states do not
+// all fit on die: MI355X has 256 MB of Infinity Cache in front of HBM
+constexpr size_t kAdjCacheBytes = size_t{192} << 20;
 constexpr int kAdjChunk = 4;    // observables whose loads are in flight together
 
 template <class T, int M>

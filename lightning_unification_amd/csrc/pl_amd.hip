@@ -1351,18 +1351,33 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
         E.obs_into(hk, lam, obs[k]);
         states.push_back(hk);
     }
+    // Observables per sweep: when lambda and every H_k exceed the on-die
+    // cache but fit in it in groups, the sweep runs once per group with a
+    // fresh lambda, so each group's states stay cache-resident across ops.
+    const size_t sbytes = E.len() * sizeof(V);
+    int per_sweep = K;
+    if (sbytes * (K + 1) > kAdjCacheBytes && sbytes * 2 <= kAdjCacheBytes) {
+        const int fit = std::max<int>(1, static_cast<int>(kAdjCacheBytes / sbytes) - 1);
+        const int nsweep = (K + fit - 1) / fit;
+        per_sweep = (K + nsweep - 1) / nsweep;
+    }
     // device tables: state pointers, jacobian, partials
     V **dstates = nullptr;
     double *djac = nullptr, *dpart = nullptr;
     const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 8), kReduceBlocks);
-    // one pointer table per chunk of kAdjMaxObs observables: lambda, then the
-    // chunk's H_k (built once, so no launch waits on a staging copy)
-    const int nchunk = (K + kAdjMaxObs - 1) / kAdjMaxObs;
-    std::vector<V *> tables(size_t(nchunk) * (kAdjMaxObs + 1), nullptr);
-    for (int c = 0; c < nchunk; c++) {
-        tables[size_t(c) * (kAdjMaxObs + 1)] = lam;
-        for (int k = c * kAdjMaxObs; k < std::min(K, (c + 1) * kAdjMaxObs); k++) {
-            tables[size_t(c) * (kAdjMaxObs + 1) + 1 + (k - c * kAdjMaxObs)] = states[1 + k];
+    // one pointer table per chunk of at most kAdjMaxObs observables of a
+    // sweep: lambda, then the chunk's H_k (built once, so no launch waits on
+    // a staging copy); chunks are numbered sweep by sweep
+    std::vector<V *> tables;
+    for (int s0 = 0; s0 < K; s0 += per_sweep) {
+        const int ns = std::min(per_sweep, K - s0);
+        for (int k0 = 0; k0 < ns; k0 += kAdjMaxObs) {
+            const size_t at = tables.size();
+            tables.resize(at + kAdjMaxObs + 1, nullptr);
+            tables[at] = lam;
+            for (int k = k0; k < std::min(ns, k0 + kAdjMaxObs); k++) {
+                tables[at + 1 + (k - k0)] = states[1 + s0 + k];
+            }
         }
     }
     HIP_OK(hipMalloc(&dstates, tables.size() * sizeof(V *)));
@@ -1384,11 +1399,17 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
     for (auto &o : ops) {
         num_par_ops += o.p.empty() ? 0 : 1;
     }
+    ScratchScope sc2(h);
+    V *mu = nullptr;
+    size_t chunk_base = 0; // first pointer table of the current sweep
+    for (int s0 = 0; s0 < K; s0 += per_sweep) {
+    const int ns = std::min(per_sweep, K - s0);
+    if (s0 > 0) {
+        E.copy(lam, E.psi()); // the next group starts again from psi
+    }
     int t_num = ntp - 1;
     long long cur_param = static_cast<long long>(num_par_ops) - 1;
     int tp_pos = ntp - 1;
-    ScratchScope sc2(h);
-    V *mu = nullptr;
     for (int i = nops - 1; i >= 0; i--) {
         const Op &op = ops[i];
         if (op.p.size() > 1) {
@@ -1414,7 +1435,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                 g = lower_generator<T>(E.n(), op.id, op.w.data(), static_cast<int>(op.w.size()),
                                        true, &s);
                 factor = -2.0 * static_cast<double>(s) * (op.inv ? -1.0 : 1.0);
-                row = t_num * K;
+                row = t_num * K + s0;
                 t_num--;
                 tp_pos--;
             }
@@ -1438,16 +1459,16 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                 E.apply(lower_generator<T>(E.n(), op.id, op.w.data(), static_cast<int>(op.w.size()),
                                            false, &s),
                         mu);
-                std::vector<double> row_vals(K);
-                for (int k = 0; k < K; k++) {
-                    row_vals[k] = factor * E.dot(states[1 + k], mu).imag();
+                std::vector<double> row_vals(ns);
+                for (int k = 0; k < ns; k++) {
+                    row_vals[k] = factor * E.dot(states[1 + s0 + k], mu).imag();
                 }
-                HIP_OK(hipMemcpyAsync(djac + row, row_vals.data(), K * sizeof(double),
+                HIP_OK(hipMemcpyAsync(djac + row, row_vals.data(), ns * sizeof(double),
                                       hipMemcpyHostToDevice, E.h->stream));
                 HIP_OK(hipStreamSynchronize(E.h->stream));
             }
-            for (int k = 0; k < K; k++) {
-                E.apply(ud, states[1 + k]);
+            for (int k = 0; k < ns; k++) {
+                E.apply(ud, states[1 + s0 + k]);
             }
             continue;
         }
@@ -1456,11 +1477,11 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
             continue;
         }
         // fused launches, kAdjMaxObs observables each; lambda updated by the last
-        for (int k0 = 0; k0 < K; k0 += kAdjMaxObs) {
-            const int nk = std::min(kAdjMaxObs, K - k0);
-            const int upd = (k0 + nk == K) ? 1 : 0;
-            // states for this launch: lambda followed by H_k0 .. H_k0+nk-1
-            V *const *st = dstates + size_t(k0 / kAdjMaxObs) * (kAdjMaxObs + 1);
+        for (int k0 = 0; k0 < ns; k0 += kAdjMaxObs) {
+            const int nk = std::min(kAdjMaxObs, ns - k0);
+            const int upd = (k0 + nk == ns) ? 1 : 0;
+            // states for this launch: lambda followed by H_(s0+k0) .. H_(s0+k0+nk-1)
+            V *const *st = dstates + (chunk_base + size_t(k0 / kAdjMaxObs)) * (kAdjMaxObs + 1);
             if (ud.kind == GroupOp<T>::Parity) {
                 const dcx<T> u0{ud.parity_phase[0].real(), ud.parity_phase[0].imag()};
                 const dcx<T> u1{ud.parity_phase[1].real(), ud.parity_phase[1].imag()};
@@ -1525,6 +1546,8 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                 HIP_OK(hipGetLastError());
             }
         }
+    }
+    chunk_base += static_cast<size_t>((ns + kAdjMaxObs - 1) / kAdjMaxObs);
     }
     std::vector<double> J(size_t(ntp) * K);
     HIP_OK(hipMemcpyAsync(J.data(), djac, J.size() * sizeof(double), hipMemcpyDeviceToHost,
