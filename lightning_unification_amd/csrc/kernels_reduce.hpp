@@ -318,11 +318,17 @@ struct PauliTileGroup {
     uint32_t xl; // x-mask in the compressed tile index (0: diagonal terms)
     int t0, t1;  // its terms
 };
+// A term of a pair group contributes s_e (cr Re z + ci Im z) for the pair
+// (e, f = e ^ x), z = conj(psi_e) psi_f and s_e = (-1)^popcount(e & zy): with
+// sigma = (-1)^popcount(x & zy) (the sign of f relative to e),
+// Re[(-i)^ny (s_e z + s_f conj z)] = s_e (cr Re z + ci Im z), where
+// cr = coeff (1 + sigma) {1, 0, -1, 0}[ny mod 4] and
+// ci = coeff (1 - sigma) {0, 1, 0, -1}[ny mod 4].  Diagonal terms: cr = coeff.
 struct PauliTileTerm {
     uint64_t zyo; // Z/Y bits outside S
-    double coeff;
+    double cr, ci;
     uint32_t zyl; // Z/Y bits inside S, compressed
-    int ny;
+    int pad;
 };
 
 template <class T>
@@ -362,38 +368,29 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
                     double s = 0;
                     for (int t = gr.t0; t < gr.t1; t++) {
                         const PauliTileTerm tm = terms[t];
-                        const int par = (__popc(e & tm.zyl) + __popcll(base & tm.zyo)) & 1;
-                        s += par ? -tm.coeff : tm.coeff;
+                        const int par = static_cast<int>((__popc(e & tm.zyl) ^ __popcll(base & tm.zyo)) & 1U);
+                        s = fma(double(1 - 2 * par), tm.cr, s);
                     }
-                    acc += p * s;
+                    acc = fma(p, s, acc);
                 }
                 continue;
             }
             const int lb = __ffs(gr.xl) - 1;
             for (uint32_t pe = threadIdx.x; pe < (tsz >> 1); pe += kBlock) {
                 const uint32_t e = ((pe >> lb) << (lb + 1)) | (pe & ((1U << lb) - 1));
-                const uint32_t f = e ^ gr.xl;
-                const v2<T> a = tile[e], c = tile[f];
+                const v2<T> a = tile[e], c = tile[e ^ gr.xl];
                 // z = conj(a) * c
                 const double zr = double(a.x) * double(c.x) + double(a.y) * double(c.y);
                 const double zi = double(a.x) * double(c.y) - double(a.y) * double(c.x);
-                double s = 0;
+                double sa = 0, sb = 0;
                 for (int t = gr.t0; t < gr.t1; t++) {
                     const PauliTileTerm tm = terms[t];
-                    const int po = __popcll(base & tm.zyo) & 1;
-                    const double si = ((__popc(e & tm.zyl) + po) & 1) ? -1.0 : 1.0;
-                    const double sj = ((__popc(f & tm.zyl) + po) & 1) ? -1.0 : 1.0;
-                    // Re[(-i)^ny (si z + sj conj z)]
-                    double v;
-                    switch (tm.ny & 3) {
-                    case 0: v = (si + sj) * zr; break;
-                    case 1: v = (si - sj) * zi; break;
-                    case 2: v = -(si + sj) * zr; break;
-                    default: v = -(si - sj) * zi; break;
-                    }
-                    s += tm.coeff * v;
+                    const int par = static_cast<int>((__popc(e & tm.zyl) ^ __popcll(base & tm.zyo)) & 1U);
+                    const double se = double(1 - 2 * par);
+                    sa = fma(se, tm.cr, sa);
+                    sb = fma(se, tm.ci, sb);
                 }
-                acc += s;
+                acc = fma(sa, zr, fma(sb, zi, acc));
             }
         }
     }

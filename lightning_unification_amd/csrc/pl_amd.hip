@@ -869,7 +869,18 @@ template <class T> struct Engine {
             for (uint64_t x : xs) {
                 PauliTileGroup gr{compress(x), static_cast<int>(ttab.size()), 0};
                 for (const auto &pt : groups[x]) {
-                    ttab.push_back(PauliTileTerm{pt.zy & ~smask, pt.coeff, compress(pt.zy & smask), pt.ny});
+                    const uint32_t zyl = compress(pt.zy & smask);
+                    double cr = pt.coeff, ci = 0;
+                    if (x != 0) { // see PauliTileTerm
+                        const double sigma = (__builtin_popcount(gr.xl & zyl) & 1) ? -1.0 : 1.0;
+                        const int ny = pt.ny & 3;
+                        cr = pt.coeff * (1 + sigma) * (ny == 0 ? 1.0 : (ny == 2 ? -1.0 : 0.0));
+                        ci = pt.coeff * (1 - sigma) * (ny == 1 ? 1.0 : (ny == 3 ? -1.0 : 0.0));
+                        if (cr == 0 && ci == 0) {
+                            continue; // contributes exactly zero
+                        }
+                    }
+                    ttab.push_back(PauliTileTerm{pt.zy & ~smask, cr, ci, zyl, 0});
                 }
                 gr.t1 = static_cast<int>(ttab.size());
                 gtab.push_back(gr);

@@ -309,7 +309,7 @@ void tile_program(int n, const TapeView &t, const TilePass &ps, int low, const s
             prog.push_back(in);
             continue;
         }
-        if (clean && g.kind == GroupOp<T>::Group && g.M <= 4 && nx == 2) {
+        if (clean && g.kind == GroupOp<T>::Group && g.M <= 4 && nx == 2 && __builtin_ctz(xo) < 6) {
             // two member bits, at least one a lane bit: the op as a 4x4 matrix U
             // over the patterns u of (mb[0], mb[1]) (identity where no member)
             int mb[2], nm = 0;
