@@ -20,6 +20,10 @@
 //            updates its own registers, indices fixed at compile time;
 //   lane     a 2x2 op whose member bit is a lane bit: one cross-lane
 //            exchange per register;
+//   mix1     a group op over one lane bit and one register bit (as a 4x4):
+//            two exchanges per register pair;
+//   mix2     a group op over two lane bits (as a 4x4): three exchanges per
+//            register;
 //   lds      any other op, applied by tp_group / tp_parity through LDS
 //            (the launch then carries a tile of LDS).
 // Control bits may be lane bits (a per-lane predicate) or register bits (a
