@@ -239,11 +239,11 @@ __device__ __forceinline__ void tr_p4(v2<T> (&v)[1 << KB], const TrInst<T> &in) 
 // One independent test per register bit (no else chain): each case keeps
 // its compile-time register indices.
 // diagonal op: v[r] *= pt[c_r], c_r = the phase-index bits register r feeds
-template <class T, int KB, int NR, int J0, int J1>
+template <class T, int KB, bool CHK, int NR, int J0, int J1>
 __device__ __forceinline__ void tr_diag(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
-        if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
             continue;
         }
         int c = 0;
@@ -257,26 +257,26 @@ __device__ __forceinline__ void tr_diag(v2<T> (&v)[1 << KB], const TrInst<T> &in
     }
 }
 
-template <class T, int KB, int J = 0>
+template <class T, int KB, bool CHK, int J = 0>
 __device__ __forceinline__ void tr_diag1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
     if constexpr (J < KB) {
         if (in.ja == J) {
-            tr_diag<T, KB, 1, J, 0>(v, in, pt);
+            tr_diag<T, KB, CHK, 1, J, 0>(v, in, pt);
         }
-        tr_diag1_at<T, KB, J + 1>(v, in, pt);
+        tr_diag1_at<T, KB, CHK, J + 1>(v, in, pt);
     }
 }
 
-template <class T, int KB, int JA = 0, int JB = 1>
+template <class T, int KB, bool CHK, int JA = 0, int JB = 1>
 __device__ __forceinline__ void tr_diag2_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
     if constexpr (JB < KB) {
         if (in.ja == JA && in.jb == JB) {
-            tr_diag<T, KB, 2, JA, JB>(v, in, pt);
+            tr_diag<T, KB, CHK, 2, JA, JB>(v, in, pt);
         }
         if constexpr (JA + 1 < JB) {
-            tr_diag2_at<T, KB, JA + 1, JB>(v, in, pt);
+            tr_diag2_at<T, KB, CHK, JA + 1, JB>(v, in, pt);
         } else {
-            tr_diag2_at<T, KB, 0, JB + 1>(v, in, pt);
+            tr_diag2_at<T, KB, CHK, 0, JB + 1>(v, in, pt);
         }
     }
 }
@@ -311,7 +311,8 @@ __device__ __forceinline__ void tr_pair_at(v2<T> (&v)[1 << KB], const TrInst<T> 
 
 // one lane member bit (lx) and one register member bit J: per register pair
 // a lane exchanges both amplitudes with lane ^ lx and forms its own two
-template <class T, int KB, int J, bool CHK>
+// GEN: the op has controls (register-bit test per pair, lane predicate `on`)
+template <class T, int KB, int J, bool GEN>
 __device__ __forceinline__ void tr_mix1(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
                                         int src, bool on) {
 #pragma unroll
@@ -319,7 +320,7 @@ __device__ __forceinline__ void tr_mix1(v2<T> (&v)[1 << KB], const TrInst<T> &in
         if ((r >> J) & 1) {
             continue;
         }
-        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+        if (GEN && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
             continue;
         }
         constexpr int b = 1 << J;
@@ -332,21 +333,26 @@ __device__ __forceinline__ void tr_mix1(v2<T> (&v)[1 << KB], const TrInst<T> &in
         tp_mac(y1, w[6], p0);
         tp_mac(y0, w[3], p1);
         tp_mac(y1, w[7], p1);
-        v[r].x = on ? y0.x : a0.x;
-        v[r].y = on ? y0.y : a0.y;
-        v[r | b].x = on ? y1.x : a1.x;
-        v[r | b].y = on ? y1.y : a1.y;
+        if (GEN) {
+            v[r].x = on ? y0.x : a0.x;
+            v[r].y = on ? y0.y : a0.y;
+            v[r | b].x = on ? y1.x : a1.x;
+            v[r | b].y = on ? y1.y : a1.y;
+        } else {
+            v[r] = y0;
+            v[r | b] = y1;
+        }
     }
 }
 
-template <class T, int KB, bool CHK, int J = 0>
+template <class T, int KB, bool GEN, int J = 0>
 __device__ __forceinline__ void tr_mix1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
                                            int src, bool on) {
     if constexpr (J < KB) {
         if (in.ja == J) {
-            tr_mix1<T, KB, J, CHK>(v, in, w, src, on);
+            tr_mix1<T, KB, J, GEN>(v, in, w, src, on);
         }
-        tr_mix1_at<T, KB, CHK, J + 1>(v, in, w, src, on);
+        tr_mix1_at<T, KB, GEN, J + 1>(v, in, w, src, on);
     }
 }
 
@@ -427,12 +433,20 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
                     pt[q].i = tc ? (b1 ? hi : li) : T(0);
                 }
             }
-            if (nr == 0) {
-                tr_diag<T, KB, 0, 0, 0>(v, in, pt);
+            if (in.rmask != 0) { // register-bit condition
+                if (nr == 0) {
+                    tr_diag<T, KB, true, 0, 0, 0>(v, in, pt);
+                } else if (nr == 1) {
+                    tr_diag1_at<T, KB, true>(v, in, pt);
+                } else {
+                    tr_diag2_at<T, KB, true>(v, in, pt);
+                }
+            } else if (nr == 0) {
+                tr_diag<T, KB, false, 0, 0, 0>(v, in, pt);
             } else if (nr == 1) {
-                tr_diag1_at<T, KB>(v, in, pt);
+                tr_diag1_at<T, KB, false>(v, in, pt);
             } else {
-                tr_diag2_at<T, KB>(v, in, pt);
+                tr_diag2_at<T, KB, false>(v, in, pt);
             }
         } else if (code == kTrParity) {
             const bool odd = __popc(static_cast<uint32_t>(lane) & in.tmask) & 1;
@@ -480,7 +494,7 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
                 w[c] = dcx<T>{hi ? hi_.r : lo_.r, hi ? hi_.i : lo_.i};
             }
             const int src = lane ^ static_cast<int>(in.lx);
-            if (in.rmask != 0) {
+            if (in.rmask != 0 || in.tmask != 0) {
                 tr_mix1_at<T, KB, true>(v, in, w, src, on);
             } else {
                 tr_mix1_at<T, KB, false>(v, in, w, src, on);
@@ -499,20 +513,34 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
                 w[d] = dcx<T>{b1 ? hr : lr, b1 ? hi : li};
             }
             const int l1 = static_cast<int>(in.lx), l2 = static_cast<int>(in.lx2);
+            if (in.rmask == 0 && in.tmask == 0) { // no controls
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
-                    continue;
+                for (int r = 0; r < R; r++) {
+                    const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
+                    const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
+                    const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
+                    v2<T> y = tp_mul(w[0], v[r]);
+                    tp_mac(y, w[1], s1);
+                    tp_mac(y, w[2], s2);
+                    tp_mac(y, w[3], s3);
+                    v[r] = y;
                 }
-                const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
-                const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
-                const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
-                v2<T> y = tp_mul(w[0], v[r]);
-                tp_mac(y, w[1], s1);
-                tp_mac(y, w[2], s2);
-                tp_mac(y, w[3], s3);
-                v[r].x = on ? y.x : v[r].x;
-                v[r].y = on ? y.y : v[r].y;
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+                        continue;
+                    }
+                    const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
+                    const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
+                    const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
+                    v2<T> y = tp_mul(w[0], v[r]);
+                    tp_mac(y, w[1], s1);
+                    tp_mac(y, w[2], s2);
+                    tp_mac(y, w[3], s3);
+                    v[r].x = on ? y.x : v[r].x;
+                    v[r].y = on ? y.y : v[r].y;
+                }
             }
         } else if (code == kTrLds) {
 #pragma unroll

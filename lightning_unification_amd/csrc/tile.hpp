@@ -230,24 +230,33 @@ void tile_program(int n, const TapeView &t, const TilePass &ps, int low, const s
         const int nx = __builtin_popcount(xo);
         if (clean && (g.M == 1 || g.diag) && g.M <= 4 && nx <= 2) {
             in.code = kTrDiag;
-            // phase index c: bit i = member bit mb[i]; mat[c] = the phase there (1: no member)
+            // phase index c: bit i = index bit mb[i]; mat[c] = the phase there
+            // (1: no member, or the condition fails).  With at most two
+            // pinned bits in all, the condition bits join the index and the
+            // condition disappears.
+            const bool fold = __builtin_popcount(pins) <= 2;
+            const uint32_t ix = fold ? pins : xo;
+            if (fold) {
+                in.tmask = in.tval = in.rmask = in.rval = 0;
+            }
             int mb[2] = {-1, -1}, nm = 0;
             for (int b = 0; b < k; b++) {
-                if ((xo >> b) & 1U) {
+                if ((ix >> b) & 1U) {
                     mb[nm++] = b;
                 }
             }
             for (int c = 0; c < 4; c++) {
-                uint32_t x = 0;
+                uint32_t u = 0;
                 for (int a = 0; a < nm; a++) {
                     if ((c >> a) & 1) {
-                        x |= 1U << mb[a];
+                        u |= 1U << mb[a];
                     }
                 }
                 in.mat[c] = dcx<T>{T(1), T(0)};
-                if (c >= (1 << nm)) {
+                if (c >= (1 << nm) || (fold && (u & cmask) != cval)) {
                     continue;
                 }
+                const uint32_t x = u & xo;
                 for (int j = 0; j < g.M; j++) {
                     if (static_cast<uint32_t>(g.off[j]) == x) {
                         in.mat[c] = cx(g.mat[j * g.M + j]);
