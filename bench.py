@@ -262,6 +262,84 @@ def bench_c1():
     return out
 
 
+def bench_next():
+    """The entry points beside the path, each timed end to end through the C ABI
+    on the state of the 20-qubit, 10-layer plain circuit (complex128):
+    - VJP (pl_vjp) of the whole tape against a seeded cotangent dy.
+      Algorithmic bytes: two states (psi, mu) swept backwards, 2 x 2 f S per
+      op, plus one read of each state per trainable parameter for the inner
+      product.
+    - <H> and Var(H) of a seeded sparse Hermitian H (pl_expval_sparse /
+      pl_var_sparse) at 22 qubits, 5 non-zeros per row (the diagonal and four
+      bit-flip partners).  The boundary takes host CSR arrays, as the
+      reference's does, so the time includes the host checks and the PCIe
+      upload of the 24 B per non-zero; the kernel alone reads those bytes
+      plus psi once.
+    - Metropolis samples (pl_generate_mcmc_samples), 10^5 shots after 100
+      burn-in steps, with both transition kernels."""
+    from lightning_unification_amd import capi
+    from lightning_unification_amd.circuits import plain_layers
+    out = {}
+    n = 20
+    S = (1 << n) * 16
+    tape, _, tp = plain_layers(n, 10, seed=1001)
+    sv = capi.DeviceStateVector(n, np.complex128)
+    sv.apply_tape(tape)
+    rng = np.random.default_rng(1003)
+    dy = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    dy /= np.linalg.norm(dy)
+    sv.vjp(tape, dy, tp)  # warm
+    t0 = time.perf_counter()
+    sv.vjp(tape, dy, tp)
+    t_vjp = time.perf_counter() - t0
+    alg = sum(2 * 2 * touched_fraction(op[0]) * S for op in tape.ops) + len(tp) * 2 * S
+    out["vjp"] = {"qubits": n, "ops": len(tape), "params": len(tp), "ms": round(t_vjp * 1e3, 2),
+                  "algorithmic_GB": round(alg / 1e9, 2), "algorithmic_GBs": round(alg / t_vjp / 1e9, 1)}
+    log(f"VJP: {len(tape)} ops, {len(tp)} params in {t_vjp*1e3:.1f} ms")
+
+    for kernel in (0, 1):
+        sv.generate_mcmc_samples(1000, kernel=kernel, num_burnin=100, seed=7)  # warm
+        t0 = time.perf_counter()
+        bits = sv.generate_mcmc_samples(100_000, kernel=kernel, num_burnin=100, seed=7)
+        t_mc = time.perf_counter() - t0
+        out[f"mcmc_kernel{kernel}"] = {"qubits": n, "shots": int(bits.shape[0]), "burnin": 100,
+                                       "ms": round(t_mc * 1e3, 2)}
+        log(f"MCMC kernel {kernel}: 1e5 shots in {t_mc*1e3:.1f} ms")
+    sv.close()
+
+    ns = 22
+    dim = 1 << ns
+    masks = np.array([0, 1, 1 << 7, 1 << 13, 1 << 21], np.int64)
+    coef = np.array([0.0, 0.5, -0.25, 0.75, 0.125])
+    rows = np.arange(dim, dtype=np.int64)
+    cols = np.sort(rows[:, None] ^ masks[None, :], axis=1)
+    diag = rng.standard_normal(dim)
+    flip = cols ^ rows[:, None]
+    vals = np.where(flip == 0, diag[:, None], 0.0)
+    for m, c in zip(masks[1:], coef[1:]):
+        vals = np.where(flip == m, c, vals)
+    entries = cols.reshape(-1)
+    values = vals.reshape(-1).astype(np.complex128)
+    row_map = np.arange(dim + 1, dtype=np.int64) * len(masks)
+    svs = capi.DeviceStateVector(ns, np.complex128)
+    svs.apply_tape(random_layers(ns, 4, seed=1004))
+    svs.expval_sparse(row_map, entries, values)  # warm
+    t0 = time.perf_counter()
+    e = svs.expval_sparse(row_map, entries, values)
+    t_e = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    v = svs.var_sparse(row_map, entries, values)
+    t_v = time.perf_counter() - t0
+    svs.close()
+    csr_bytes = entries.size * 24 + row_map.size * 8
+    out["sparse_expval_var"] = {"qubits": ns, "nnz": int(entries.size), "expval": e, "var": v,
+                                "expval_ms": round(t_e * 1e3, 2), "var_ms": round(t_v * 1e3, 2),
+                                "csr_GB": round(csr_bytes / 1e9, 3),
+                                "includes": "host checks and PCIe upload of the CSR arrays"}
+    log(f"sparse H at {ns} qubits, nnz {entries.size}: <H> {t_e*1e3:.1f} ms, var {t_v*1e3:.1f} ms")
+    return out
+
+
 def bench_tiled(sv_ref, tape, n, reps):
     """The same circuit through the tile-pass executor (pl_sv_apply_ops_tiled,
     default tile shape): `reps` applications to a fresh |0>, each timed with
@@ -438,7 +516,9 @@ def main():
                     help="training tuples of the learned-surrogate form (0 skips it)")
     ap.add_argument("--no-c1", action="store_true", help="skip the 20-qubit adjoint measurement")
     ap.add_argument("--no-tiled", action="store_true", help="skip the tile-pass measurement")
-    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate", "c1"],
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the VJP, sparse-H and Metropolis measurements")
+    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate", "c1", "next"],
                     default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
@@ -465,6 +545,9 @@ def main():
         return
     if args.mode == "c1":
         print(json.dumps({"mode": "c1", "c1": bench_c1()}))
+        return
+    if args.mode == "next":
+        print(json.dumps({"mode": "next", "next_rows": bench_next()}))
         return
     if args.mode == "surrogate":
         print(json.dumps({"mode": "surrogate", "surrogate": bench_surrogate(args.surrogate)}))
@@ -538,6 +621,7 @@ def main():
     sv.close()
 
     c1 = None if args.no_c1 else bench_c1()
+    next_rows = None if args.no_next else bench_next()
     batched = bench_batch(args.batch, args.plain_s) if args.batch > 0 else None
     precision = bench_precision(args.precision) if args.precision > 0 else None
     surrogate = bench_surrogate(args.surrogate) if args.surrogate > 0 else None
@@ -571,6 +655,7 @@ def main():
         "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
         "tiled_circuit": tiled,
         "c1_adjoint": c1,
+        "next_rows": next_rows,
         "batched": batched,
         "precision": precision,
         "surrogate": surrogate,
