@@ -781,6 +781,16 @@ template <class T> struct Engine {
         return {r.x, r.y};
     }
 
+    // <a|b> left in device memory at *slot, ordered on the engine's stream with
+    // no wait; part holds kReduceBlocks partials and must not overlap slot.
+    void dot_to(const V *a, const V *b, double2 *part, double2 *slot) {
+        const unsigned grid = std::min<unsigned>(grid_for(len(), 1, 4), kReduceBlocks);
+        hipLaunchKernelGGL((k_dot<T>), dim3(grid), dim3(kBlock), 0, h->stream, a, b, len(), part);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, (int)grid, slot);
+        HIP_OK(hipGetLastError());
+    }
+
     // sum_t c_t <psi|P_t|psi> over Pauli words, grouped by x-mask
     // sum_t c_t <psi|P_t|psi> over Pauli words.  Distinct x-masks are packed
     // greedily into passes whose high bits fit one LDS tile (k_pauli_tile);
@@ -1622,6 +1632,12 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
     }
     long long cur_param = static_cast<long long>(num_par_ops) - 1;
     int tp_pos = ntp - 1;
+    // <G mu|lambda> of trainable parameter t stays on the device in dots[t]
+    // until the sweep ends: one read-back instead of a wait per parameter
+    auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + size_t(ntp)) * sizeof(double2)));
+    double2 *dots = part + kReduceBlocks;
+    std::vector<double> scale(ntp, 0.0);
+    std::vector<char> reached(ntp, 0);
     for (int i = nops - 1; i >= 0; i--) {
         if (nparams[i] > 1) {
             throw InvalidArg("The operation is not supported using the adjoint differentiation method");
@@ -1638,10 +1654,9 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
                 E.copy(mud, mu);
                 T s = 0;
                 E.apply(lower_generator<T>(E.n(), ids[i], w, nwires[i], false, &s), mud);
-                const double sc_f = static_cast<double>(s) * (inverse[i] ? -1.0 : 1.0);
-                const std::complex<double> z = E.dot(mud, lam);
-                out[2 * tp_pos] = -sc_f * z.imag();
-                out[2 * tp_pos + 1] = sc_f * z.real();
+                scale[tp_pos] = static_cast<double>(s) * (inverse[i] ? -1.0 : 1.0);
+                reached[tp_pos] = 1;
+                E.dot_to(mud, lam, part, dots + tp_pos);
                 tp_pos--;
             }
             cur_param--;
@@ -1652,7 +1667,15 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
         E.apply(ud, lam);
         E.apply(ud, mu);
     }
+    std::vector<double2> z(ntp);
+    HIP_OK(hipMemcpyAsync(z.data(), dots, ntp * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
+    for (int t = 0; t < ntp; t++) {
+        if (reached[t]) {
+            out[2 * t] = -scale[t] * z[t].y;
+            out[2 * t + 1] = scale[t] * z[t].x;
+        }
+    }
 }
 
 extern "C" {
