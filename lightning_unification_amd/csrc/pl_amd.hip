@@ -2123,12 +2123,19 @@ static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_si
     }
     dispatch_or_throw(h, [&](auto &E) {
         using T = typename std::remove_reference_t<decltype(E)>::Scalar;
-        std::vector<dcx<T>> vals(std::max<int64_t>(nnz, 1));
-        for (int64_t j = 0; j < nnz; j++) {
-            vals[j] = {static_cast<T>(values[2 * j]), static_cast<T>(values[2 * j + 1])};
+        // complex128 values are uploaded as given; complex64 ones are rounded first
+        std::vector<dcx<T>> vals;
+        const void *vsrc = values;
+        if constexpr (!std::is_same_v<T, double>) {
+            vals.resize(std::max<int64_t>(nnz, 1));
+            for (int64_t j = 0; j < nnz; j++) {
+                vals[j] = {static_cast<T>(values[2 * j]), static_cast<T>(values[2 * j + 1])};
+            }
+            vsrc = vals.data();
         }
         unsigned char *dev = nullptr;
-        const size_t rb = row_map_size * 8, eb = std::max<int64_t>(nnz, 1) * 8, vb = vals.size() * sizeof(dcx<T>);
+        const size_t rb = row_map_size * 8, eb = std::max<int64_t>(nnz, 1) * 8,
+                     vb = std::max<int64_t>(nnz, 1) * sizeof(dcx<T>);
         HIP_OK(hipMalloc(&dev, rb + eb + vb));
         struct Free {
             unsigned char *p;
@@ -2138,7 +2145,9 @@ static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_si
         if (nnz > 0) {
             HIP_OK(hipMemcpyAsync(dev + rb, entries, nnz * 8, hipMemcpyHostToDevice, h->stream));
         }
-        HIP_OK(hipMemcpyAsync(dev + rb + eb, vals.data(), vb, hipMemcpyHostToDevice, h->stream));
+        if (nnz > 0) {
+            HIP_OK(hipMemcpyAsync(dev + rb + eb, vsrc, nnz * sizeof(dcx<T>), hipMemcpyHostToDevice, h->stream));
+        }
         const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 4), kReduceBlocks);
         auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + 1) * sizeof(double2)));
         hipLaunchKernelGGL((k_sparse_moments<T>), dim3(grid), dim3(kBlock), 0, h->stream, E.psi(), E.len(),
