@@ -8,7 +8,8 @@ that can change a fraction f of the amplitudes moves 2 * f * 2^n * 16 bytes.
 Every launch is bracketed by HIP events on the engine's stream (the stream
 the kernels run on).  Also reported: the 100-term Hamiltonian expectation and
 10^6 inverse-CDF samples on the final state, the same circuit through the
-tile-pass executor (many gates per read and write of the state), and a CPU
+tile-pass executor (many gates per read and write of the state), the VJP,
+sparse-Hamiltonian and Metropolis entry points (next_rows), and a CPU
 baseline (the reference's Lightning-Qubit kernels compiled from source, or the
 oracle restatement) on a bounded sample of the same gates.
 
