@@ -499,6 +499,49 @@ def bench_surrogate(samples=1 << 20, epochs=3):
     return out
 
 
+def bench_stream(n=30, reps=10):
+    """Streaming ceilings of the device for complex128 amplitudes at the
+    headline state size (16 GiB): read only, write only, in-place
+    read-modify-write, and the pair pattern of a dense 1-qubit gate on target
+    bit p (the access pattern of k_group_flat<double, 2>), with the engine's
+    non-temporal loads and stores (csrc/stream_probe.hip).  GB/s count the
+    bytes each kernel must move: 16 B per amplitude read and/or written."""
+    import ctypes
+    from lightning_unification_amd import capi
+    lib = ctypes.CDLL(os.path.join(ROOT, "lightning_unification_amd", "libpl_stream.so"))
+    lib.sp_run.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
+    lib.sp_last_error.restype = ctypes.c_char_p
+    sv = capi.DeviceStateVector(n, np.complex128)
+    scratch = capi.DeviceStateVector(max(1, n - 8 - 1), np.complex128)  # >= one double per workgroup
+    dev, sdev = sv.device_ptr(), scratch.device_ptr()
+    amps = 1 << n
+    out = {"qubits": n, "state_GiB": amps * 16 / 2 ** 30, "reps": reps}
+
+    def run(mode, p=0):
+        ms = ctypes.c_double()
+        lib.sp_run(ctypes.c_void_p(dev), amps, mode, p, 1, ctypes.c_void_p(sdev), ctypes.byref(ms))  # warm
+        rc = lib.sp_run(ctypes.c_void_p(dev), amps, mode, p, reps, ctypes.c_void_p(sdev), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(lib.sp_last_error().decode())
+        return ms.value
+
+    for name, mode, mult in (("read", 0, 1), ("write", 1, 1), ("rmw", 2, 2)):
+        ms = run(mode)
+        out[name] = {"ms": round(ms, 4), "GBs": round(mult * amps * 16 / (ms / 1e3) / 1e9, 1)}
+        log(f"stream {name}: {ms:.3f} ms, {out[name]['GBs']} GB/s")
+    pairs = {}
+    for p in (0, 3, 6, 8, 12, 16, 20, 25, 29):
+        if p < n:
+            ms = run(3, p)
+            pairs[str(p)] = {"ms": round(ms, 4), "GBs": round(2 * amps * 16 / (ms / 1e3) / 1e9, 1)}
+            log(f"stream pair p={p}: {ms:.3f} ms, {pairs[str(p)]['GBs']} GB/s")
+    out["rmw_pair_by_target_bit"] = pairs
+    sv.close()
+    scratch.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,7 +562,8 @@ def main():
     ap.add_argument("--no-tiled", action="store_true", help="skip the tile-pass measurement")
     ap.add_argument("--no-next", action="store_true",
                     help="skip the VJP, sparse-H and Metropolis measurements")
-    ap.add_argument("--mode", choices=["gates", "copy", "batch", "precision", "surrogate", "c1", "next"],
+    ap.add_argument("--mode", choices=["gates", "copy", "stream", "batch", "precision", "surrogate", "c1",
+                                       "next"],
                     default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
@@ -537,6 +581,9 @@ def main():
             b.copy_from(a)
         b.synchronize()
         print(json.dumps({"mode": "copy", "bytes_per_copy": (1 << 28) * 16, "copies": 3}))
+        return
+    if args.mode == "stream":
+        print(json.dumps({"mode": "stream", "stream": bench_stream(args.qubits)}))
         return
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", "batched": bench_batch(args.batch, args.plain_s)}))

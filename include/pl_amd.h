@@ -75,8 +75,17 @@ int pl_sv_dtype(pl_sv_t sv, int *out);
 int pl_sv_upload(pl_sv_t sv, const void *host, size_t nbytes);
 /* Device -> host copy (getDataVector / reading the numpy buffer). */
 int pl_sv_download(pl_sv_t sv, void *host, size_t nbytes);
-/* dst <- src (copy constructor, StateVectorLQubitManaged.hpp:89-94). */
+/* dst <- src (copy constructor, StateVectorLQubitManaged.hpp:89-94).  The
+ * copy is queued on dst's stream after src's queued work, and src's stream
+ * waits for it, so work queued on either handle afterwards sees the copy
+ * complete. */
 int pl_sv_copy(pl_sv_t dst, pl_sv_t src);
+/* Bytes moved host -> device by pl_sv_upload and device -> host by
+ * pl_sv_download since the library was loaded (either pointer may be NULL).
+ * No reference counterpart: Lightning-Qubit's state is host memory.  It lets
+ * the drop-in module show that re-wrapping an unchanged, device-mirrored
+ * buffer moves no bytes. */
+int pl_transfer_counters(uint64_t *h2d_bytes, uint64_t *d2h_bytes);
 /* |0...0> in place. */
 int pl_sv_reset(pl_sv_t sv);
 /* Basis state |index> in place (BasisState preparation). */
@@ -164,10 +173,13 @@ int pl_var_sparse(pl_sv_t sv, const int64_t *row_map, int64_t row_map_size,
  * uniformly chosen bit to a uniformly chosen value.  (The reference's
  * kernel_factory builds kernel 0 for the name "Local" and kernel 1 for
  * "NonZeroRandom"; the pybind layer keeps that mapping.)  nchains independent
- * chains (0 = automatic) each start at index 0, discard num_burnin steps and
- * record consecutive states, filling the output in chain order; SplitMix64
- * streams keyed by seed.  The reference's single chain is unseedable, so
- * results agree in distribution only. */
+ * chains (0 = automatic: one per shot up to 16384) each start from an
+ * inverse-CDF draw of |psi|^2 (the reference's one chain starts at index 0;
+ * a chain started in the stationary distribution needs no burn-in to be
+ * unbiased, which short chains rely on), discard num_burnin steps and record
+ * consecutive states, filling the output in chain order; SplitMix64 streams
+ * keyed by seed.  The reference's single chain is unseedable, so results
+ * agree in distribution only. */
 int pl_generate_mcmc_samples(pl_sv_t sv, int kernel, int64_t num_burnin,
                              int64_t nshots, uint64_t seed, int nchains,
                              uint64_t *bits);

@@ -37,9 +37,67 @@ def _run(cmd, verbose):
     return r
 
 
-def build_engine(verbose=False):
+# (source, object, extra flags): the tile-pass kernel once per
+# (precision, tile size) so that its variants compile in parallel
+ENGINE_UNITS = [("pl_amd.hip", "pl_amd.o", []), ("batch.hip", "batch.o", []), ("mlp.hip", "mlp.o", [])] + [
+    ("tiled.hip", f"tiled_{'c128' if d else 'c64'}_{kb}.o", [f"-DPL_TILE_DOUBLE={d}", f"-DPL_TILE_KB={kb}"])
+    for d in (1, 0) for kb in (3, 4, 5, 6)]
+
+
+def _deps(path, seen=None):
+    """The file and every header it includes with quotes, recursively."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("#include \""):
+                inc = line.split('"')[1]
+                _deps(os.path.normpath(os.path.join(os.path.dirname(path), inc)), seen)
+    return seen
+
+
+def build_engine(verbose=False, incremental=False):
+    """Each translation unit compiles on its own, all in parallel, then one
+    link.  incremental=True skips units whose object is newer than every file
+    they include (for development loops; build() always compiles)."""
     out = os.path.join(HERE, "libpl_amd.so")
-    _run([HIPCC, *HIP_FLAGS, "-o", out, os.path.join(CSRC, "pl_amd.hip")], verbose)
+    objdir = os.path.join(ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    procs, objs = [], []
+    for unit, obj, extra in ENGINE_UNITS:
+        src = os.path.join(CSRC, unit)
+        obj = os.path.join(objdir, obj)
+        objs.append(obj)
+        if incremental and os.path.exists(obj):
+            newest = max(os.path.getmtime(d) for d in _deps(src))
+            if os.path.getmtime(obj) > max(newest, os.path.getmtime(__file__)):
+                continue
+        cmd = [HIPCC, *compile_flags, *extra, "-c", "-o", obj, src]
+        if verbose:
+            print("+", " ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+    failed = None
+    for cmd, pr in procs:
+        text, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(text)
+            failed = failed or cmd
+    if failed:
+        raise RuntimeError(f"build step failed: {failed[-1]}")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs], verbose)
+    return out
+
+
+def build_stream_probe(verbose=False):
+    """libpl_stream.so: the device streaming probe the benchmark's stream mode
+    loads (not part of the engine)."""
+    out = os.path.join(HERE, "libpl_stream.so")
+    _run([HIPCC, *HIP_FLAGS, "-o", out, os.path.join(CSRC, "stream_probe.hip")], verbose)
     return out
 
 
@@ -67,5 +125,6 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_engine(verbose)
+    build_stream_probe(verbose)
     build_bindings(verbose)
     build_oracle(verbose)

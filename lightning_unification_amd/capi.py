@@ -32,6 +32,7 @@ EXPORTS = [
     "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
     "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train", "pl_vjp", "pl_expval_sparse",
     "pl_var_sparse", "pl_generate_mcmc_samples", "pl_sv_apply_ops_tiled", "pl_tile_plan",
+    "pl_transfer_counters",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -75,6 +76,13 @@ def load():
                                      ctypes.c_void_p]
         _lib = lib
     return _lib
+
+
+def transfer_counters():
+    """(bytes uploaded, bytes downloaded) by state uploads / downloads so far."""
+    h2d, d2h = ctypes.c_uint64(), ctypes.c_uint64()
+    _check(load().pl_transfer_counters(ctypes.byref(h2d), ctypes.byref(d2h)))
+    return h2d.value, d2h.value
 
 
 def _p(a):
@@ -156,6 +164,12 @@ class DeviceStateVector:
         out = np.empty(self.size, self.dtype)
         _check(load().pl_sv_download(self.h, _p(out), out.nbytes))
         return out
+
+    def device_ptr(self):
+        """Device address of the amplitudes (owned by this object)."""
+        out = ctypes.c_void_p()
+        _check(load().pl_sv_device_ptr(self.h, ctypes.byref(out)))
+        return out.value
 
     def copy_from(self, other):
         _check(load().pl_sv_copy(self.h, other.h))

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ObservablesAMD.hpp"
@@ -117,14 +118,16 @@ template <class StateVectorT> class Measurements final {
         return idx;
     }
     /* generate_samples_metropolis(kernelname, num_burnin, num_samples):
-     * MeasurementsLQubit.hpp:388-434, with the reference's kernel_factory
-     * mapping of names ("Local" -> proposals uniform over the non-zero
-     * amplitudes, otherwise single-bit flips; TransitionKernels.hpp). */
+     * MeasurementsLQubit.hpp:388-434, with the reference's name mapping:
+     * "NonZeroRandom" selects TransitionKernelType::NonZeroRandom, every
+     * other name TransitionKernelType::Local (:396-400), and kernel_factory
+     * (TransitionKernels.hpp:149-159) builds single-bit flips for the former
+     * and proposals uniform over the non-zero amplitudes for the latter. */
     std::vector<size_t> generate_samples_metropolis(const std::string &kernelname, size_t num_burnin,
                                                     size_t num_samples) {
         const size_t n = sv_.getNumQubits();
         std::vector<uint64_t> bits(num_samples * n);
-        const int kind = kernelname == "Local" ? 0 : 1;
+        const int kind = kernelname == "NonZeroRandom" ? 1 : 0;
         check(pl_generate_mcmc_samples(sv_.handle(), kind, static_cast<int64_t>(num_burnin),
                                        static_cast<int64_t>(num_samples), next_seed(num_samples), 0,
                                        bits.data()));
@@ -143,19 +146,34 @@ template <class StateVectorT> class Measurements final {
         }
         return seed;
     }
+    /* The C-ABI takes int64 indices and interleaved double values.  When the
+     * caller's arrays already have that layout (complex128 values,
+     * 64-bit indices; std::complex<double> is a pair of doubles) they are
+     * passed through as they are; otherwise they are converted once. */
     template <class F, class index_type>
     PrecisionT sparse(F fn, const index_type *row_map_ptr, const index_type row_map_size,
                       const index_type *entries_ptr, const ComplexT *values_ptr, const index_type numNNZ) {
-        std::vector<int64_t> rm(row_map_ptr, row_map_ptr + row_map_size), en(entries_ptr, entries_ptr + numNNZ);
-        std::vector<double> va(2 * static_cast<size_t>(numNNZ) + 2);
-        for (index_type j = 0; j < numNNZ; j++) {
-            va[2 * j] = values_ptr[j].real();
-            va[2 * j + 1] = values_ptr[j].imag();
-        }
-        en.push_back(0);
         double out = 0;
-        check(fn(sv_.handle(), rm.data(), static_cast<int64_t>(rm.size()), en.data(), va.data(),
-                 static_cast<int64_t>(numNNZ), &out));
+        if constexpr (std::is_same_v<PrecisionT, double> && std::is_integral_v<index_type> &&
+                      sizeof(index_type) == sizeof(int64_t)) {
+            static const int64_t no_entry = 0;
+            static const double no_value[2] = {0, 0};
+            const auto *en = numNNZ > 0 ? reinterpret_cast<const int64_t *>(entries_ptr) : &no_entry;
+            const auto *va = numNNZ > 0 ? reinterpret_cast<const double *>(values_ptr) : no_value;
+            check(fn(sv_.handle(), reinterpret_cast<const int64_t *>(row_map_ptr),
+                     static_cast<int64_t>(row_map_size), en, va, static_cast<int64_t>(numNNZ), &out));
+        } else {
+            std::vector<int64_t> rm(row_map_ptr, row_map_ptr + row_map_size),
+                en(entries_ptr, entries_ptr + numNNZ);
+            std::vector<double> va(2 * static_cast<size_t>(numNNZ) + 2);
+            for (index_type j = 0; j < numNNZ; j++) {
+                va[2 * j] = values_ptr[j].real();
+                va[2 * j + 1] = values_ptr[j].imag();
+            }
+            en.push_back(0);
+            check(fn(sv_.handle(), rm.data(), static_cast<int64_t>(rm.size()), en.data(), va.data(),
+                     static_cast<int64_t>(numNNZ), &out));
+        }
         return static_cast<PrecisionT>(out);
     }
 

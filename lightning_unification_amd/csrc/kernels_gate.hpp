@@ -90,7 +90,10 @@ __device__ __forceinline__ uint64_t xcd_block() {
 }
 
 // One-shot group op: thread per group, grid covers the work exactly once.
-template <class T, int M>
+// DIAG (a diagonal matrix: each member scales itself) is a template
+// parameter, so that each instantiation has straight-line loads and stores
+// and every store keeps its non-temporal hint.
+template <class T, int M, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_group_flat(v2<T> *__restrict__ sv, uint64_t nwork,
                                                        const SmallOp<T, M> op) {
     const uint64_t g = xcd_block() * kBlock + threadIdx.x;
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_group_flat(v2<T> *__restrict__ sv, u
     for (int j = 0; j < M; j++) {
         v[j] = ld_nt<T>(sv + (base | op.off[j]));
     }
-    if (op.diag) {
+    if constexpr (DIAG) {
 #pragma unroll
         for (int j = 0; j < M; j++) {
             st_nt<T>(sv + (base | op.off[j]), cmul(op.mat[j * M + j], v[j]));
@@ -125,10 +128,10 @@ __global__ __launch_bounds__(kBlock) void k_group_flat(v2<T> *__restrict__ sv, u
 // one amplitude (fully coalesced wavefront loads), its partner lives in lane
 // l ^ (1 << p) and is fetched with a cross-lane shuffle.  `geo` pins the
 // control bits only; the target bit is a free bit of the thread index.
-template <class T>
+template <class T, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_pair_lane(v2<T> *__restrict__ sv, uint64_t nelem,
                                                       Geom geo, int p, dcx<T> m00, dcx<T> m01,
-                                                      dcx<T> m10, dcx<T> m11, int diag) {
+                                                      dcx<T> m10, dcx<T> m11) {
     const uint64_t t = xcd_block() * kBlock + threadIdx.x;
     // nelem is a multiple of 64 here, so whole wavefronts are in or out
     if (t >= nelem) {
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_lane(v2<T> *__restrict__ sv, ui
     o.y = __shfl_xor(v.y, 1 << p, 64);
     const bool hi = (t >> p) & 1;
     v2<T> r;
-    if (diag) {
+    if constexpr (DIAG) {
         r = cmul(hi ? m11 : m00, v);
     } else if (hi) {
         r = cmul(m10, o);
@@ -175,7 +178,7 @@ template <class T, int M> struct TileOp {
 };
 
 // NHI = number of high pins (op.hi.nb); the LDS tile holds 2^NHI * 256 amplitudes.
-template <class T, int M, int NHI>
+template <class T, int M, int NHI, bool DIAG>
 __global__ __launch_bounds__(kBlock) void k_group_tile(v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const TileOp<T, M> op) {
     __shared__ v2<T> tile[(1 << NHI) << kTileLow];
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_group_tile(v2<T> *__restrict__ sv, u
         for (int j = 0; j < M; j++) {
             v[j] = tile[((op.fixed_h | op.off_h[j]) << kTileLow) | (be | op.off_e[j])];
         }
-        if (op.diag) {
+        if constexpr (DIAG) {
 #pragma unroll
             for (int j = 0; j < M; j++) {
                 tile[((op.fixed_h | op.off_h[j]) << kTileLow) | (be | op.off_e[j])] =

@@ -2,10 +2,12 @@
 // MeasurementsLQubit.hpp:388-434, metropolis_step :613-640, proposals of
 // TransitionKernels.hpp).
 //
-// The reference runs one sequential chain.  Here many independent chains run
-// at once, one per thread: chain c starts at index 0, discards `burnin`
-// steps, then records its next `per_chain` states as output rows
-// c * per_chain ... (the last chain is cut at the shot count).  A step
+// The reference runs one sequential chain from index 0.  Here many
+// independent chains run at once, one per thread: chain c starts at
+// start[c], an inverse-CDF draw from |psi|^2 (so every chain is in the
+// stationary distribution from its first step, however short it is),
+// discards `burnin` steps, then records its next `per_chain` states as output
+// rows c * per_chain ... (the last chain is cut at the shot count).  A step
 // proposes an index, reads |psi|^2 there and accepts with probability
 // min(1, p_new / p_old) (always when p_old = 0, as exp(log p_new - log 0)
 // does in the reference).  Proposal kinds:
@@ -91,7 +93,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_mcmc_chains(const v2<T> *__restrict__ sv, uint64_t len,
                                                         int n, int kind,
                                                         const uint64_t *__restrict__ nz, uint64_t nnz,
-                                                        uint64_t burnin, uint64_t per_chain,
+                                                        const int64_t *__restrict__ start, uint64_t burnin, uint64_t per_chain,
                                                         uint64_t nshots, uint64_t seed,
                                                         int64_t *__restrict__ idx_out) {
     const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -101,8 +103,8 @@ __global__ __launch_bounds__(kBlock) void k_mcmc_chains(const v2<T> *__restrict_
     }
     const uint64_t rec = per_chain < nshots - first ? per_chain : nshots - first;
     McmcRng rng{splitmix_mix(seed + (c + 1) * 0x9E3779B97F4A7C15ull)};
-    uint64_t idx = 0;
-    const v2<T> a0 = sv[0];
+    uint64_t idx = static_cast<uint64_t>(start[c]);
+    const v2<T> a0 = sv[idx];
     double p = double(a0.x) * a0.x + double(a0.y) * a0.y;
     for (uint64_t st = 0; st < burnin + rec; st++) {
         uint64_t t;

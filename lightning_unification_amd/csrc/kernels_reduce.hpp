@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(const v2<T> *__restrict__ a,
 }
 
 // Sum `count` partials (fixed order) into out[0]; one block.
-__global__ __launch_bounds__(kBlock) void k_reduce_final(const double2 *__restrict__ partial,
+static __global__ __launch_bounds__(kBlock) void k_reduce_final(const double2 *__restrict__ partial,
                                                          int count, double2 *__restrict__ out) {
     double2 acc{0, 0};
     for (int i = threadIdx.x; i < count; i += kBlock) {
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_marginal(const v2<T> *__restrict__ s
 }
 
 // Final per-bin sum over `chunks` partials (thread per bin).
-__global__ __launch_bounds__(kBlock) void k_marginal_final(const double *__restrict__ partial,
+static __global__ __launch_bounds__(kBlock) void k_marginal_final(const double *__restrict__ partial,
                                                            uint64_t nbins, int chunks,
                                                            double *__restrict__ out) {
     const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -331,64 +331,119 @@ struct PauliTileTerm {
     int pad;
 };
 
+// One pass of the Pauli-sum expectation over the whole state.  Persistent
+// workgroups walk the tiles q = blockIdx.x, blockIdx.x + gridDim.x, ...; each
+// thread holds its share of the next tile in registers while the current one
+// is evaluated from LDS, so the state streams in behind the arithmetic.
+// Per tile: the terms' parities over the bits outside S are constant and are
+// folded into the sign of (cr, ci) once; per pair and term one AND, one
+// popcount and two FMAs remain.
+constexpr int kPauliPerThread = 8;              // amplitudes a thread loads per tile (2^11 / 256)
+constexpr int kPauliMaxTerms = 128;             // terms of one pass held in LDS
+struct PauliPassTerm {
+    double cr, ci;
+    uint32_t zyl;
+    int pad;
+};
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const PauliTileGeom geo, int ngroups,
                                                        const PauliTileGroup *__restrict__ groups,
-                                                       const PauliTileTerm *__restrict__ terms,
-                                                       double2 *__restrict__ partial) {
+                                                       const PauliTileTerm *__restrict__ terms, int t_first,
+                                                       int nterms, double2 *__restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
+    __shared__ PauliPassTerm tt[kPauliMaxTerms];
     const int nS = geo.nlow + geo.nhigh;
     const uint32_t tsz = 1U << nS;
     const uint32_t lowmask = (1U << geo.nlow) - 1;
-    double acc = 0;
-    for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
-        // base: q spread over the bits outside S (insert zeros at S, ascending)
+    const int per = static_cast<int>((tsz + kBlock - 1) / kBlock); // <= kPauliPerThread
+    // the tile-local part of each loaded element's state index
+    uint64_t off[kPauliPerThread];
+#pragma unroll
+    for (int k = 0; k < kPauliPerThread; k++) {
+        const uint32_t e = threadIdx.x + k * kBlock;
+        uint64_t g = e & lowmask;
+        for (int b = 0; b < geo.nhigh; b++) {
+            g |= static_cast<uint64_t>((e >> (geo.nlow + b)) & 1U) << geo.high[b];
+        }
+        off[k] = g;
+    }
+    // q spread over the bits outside S (zeros inserted at S, ascending)
+    auto tile_base = [&](uint64_t q) {
         uint64_t base = q << geo.nlow;
         for (int k = 0; k < geo.nhigh; k++) {
             const int p = geo.high[k];
             base = ((base >> p) << (p + 1)) | (base & ((uint64_t{1} << p) - 1));
         }
-        __syncthreads(); // the previous tile is no longer read
-        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
-            uint64_t g = base | (e & lowmask);
-            for (int k = 0; k < geo.nhigh; k++) {
-                g |= static_cast<uint64_t>((e >> (geo.nlow + k)) & 1U) << geo.high[k];
+        return base;
+    };
+    v2<T> nxt[kPauliPerThread];
+    uint64_t q = blockIdx.x;
+    uint64_t base = tile_base(q);
+    if (q < ntiles) {
+#pragma unroll
+        for (int k = 0; k < kPauliPerThread; k++) {
+            if (k < per && threadIdx.x + k * kBlock < tsz) {
+                nxt[k] = ld_nt<T>(&sv[base | off[k]]);
             }
-            tile[e] = sv[g];
+        }
+    }
+    double acc = 0;
+    for (; q < ntiles; q += gridDim.x) {
+        __syncthreads(); // the previous tile and its term table are no longer read
+#pragma unroll
+        for (int k = 0; k < kPauliPerThread; k++) {
+            if (k < per && threadIdx.x + k * kBlock < tsz) {
+                tile[threadIdx.x + k * kBlock] = nxt[k];
+            }
+        }
+        for (int t = threadIdx.x; t < nterms; t += kBlock) {
+            const PauliTileTerm tm = terms[t_first + t];
+            const bool neg = (__popcll(base & tm.zyo) & 1U) != 0;
+            tt[t] = PauliPassTerm{neg ? -tm.cr : tm.cr, neg ? -tm.ci : tm.ci, tm.zyl, 0};
         }
         __syncthreads();
+        const uint64_t qn = q + gridDim.x;
+        if (qn < ntiles) { // next tile's loads in flight during this tile's arithmetic
+            base = tile_base(qn);
+#pragma unroll
+            for (int k = 0; k < kPauliPerThread; k++) {
+                if (k < per && threadIdx.x + k * kBlock < tsz) {
+                    nxt[k] = ld_nt<T>(&sv[base | off[k]]);
+                }
+            }
+        }
         for (int gi = 0; gi < ngroups; gi++) {
             const PauliTileGroup gr = groups[gi];
+            const int t0 = gr.t0 - t_first, t1 = gr.t1 - t_first;
             if (gr.xl == 0) {
                 for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
                     const v2<T> a = tile[e];
                     const double p = double(a.x) * double(a.x) + double(a.y) * double(a.y);
                     double s = 0;
-                    for (int t = gr.t0; t < gr.t1; t++) {
-                        const PauliTileTerm tm = terms[t];
-                        const int par = static_cast<int>((__popc(e & tm.zyl) ^ __popcll(base & tm.zyo)) & 1U);
-                        s = fma(double(1 - 2 * par), tm.cr, s);
+                    for (int t = t0; t < t1; t++) {
+                        const double c = (__popc(e & tt[t].zyl) & 1U) ? -tt[t].cr : tt[t].cr;
+                        s += c;
                     }
                     acc = fma(p, s, acc);
                 }
                 continue;
             }
             const int lb = __ffs(gr.xl) - 1;
+            const uint32_t lowm = (1U << lb) - 1;
             for (uint32_t pe = threadIdx.x; pe < (tsz >> 1); pe += kBlock) {
-                const uint32_t e = ((pe >> lb) << (lb + 1)) | (pe & ((1U << lb) - 1));
+                const uint32_t e = ((pe & ~lowm) << 1) | (pe & lowm);
                 const v2<T> a = tile[e], c = tile[e ^ gr.xl];
                 // z = conj(a) * c
                 const double zr = double(a.x) * double(c.x) + double(a.y) * double(c.y);
                 const double zi = double(a.x) * double(c.y) - double(a.y) * double(c.x);
                 double sa = 0, sb = 0;
-                for (int t = gr.t0; t < gr.t1; t++) {
-                    const PauliTileTerm tm = terms[t];
-                    const int par = static_cast<int>((__popc(e & tm.zyl) ^ __popcll(base & tm.zyo)) & 1U);
-                    const double se = double(1 - 2 * par);
-                    sa = fma(se, tm.cr, sa);
-                    sb = fma(se, tm.ci, sb);
+                for (int t = t0; t < t1; t++) {
+                    const bool neg = (__popc(e & tt[t].zyl) & 1U) != 0;
+                    sa += neg ? -tt[t].cr : tt[t].cr;
+                    sb += neg ? -tt[t].ci : tt[t].ci;
                 }
                 acc = fma(sa, zr, fma(sb, zi, acc));
             }

@@ -1,274 +1,23 @@
-// C-ABI of the gfx950 state-vector engine (declared in include/pl_amd.h).
+// C-ABI of the gfx950 state-vector engine (declared in include/pl_amd.h):
+// state lifecycle, gates, measurements, sampling, the adjoint and VJP sweeps,
+// the fused and tiled executors.  The batched-circuit and MLP entry points
+// are in batch.hip and mlp.hip.
 //
 // Host orchestration: every entry point validates its arguments, lowers the
 // request (lower.hpp) and enqueues kernels (kernels_*.hpp) on the handle's
 // HIP stream.  There is no CPU compute path: if the device work cannot be
 // enqueued the call fails with PL_ERR_DEVICE.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <complex>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <numeric>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <limits>
-#include <vector>
-
-#include "../../include/pl_amd.h"
+#include "engine_common.hpp"
 #include "kernels_adjoint.hpp"
-#include "kernels_gate.hpp"
 #include "kernels_reduce.hpp"
 #include "kernels_sample.hpp"
 #include "kernels_mfma.hpp"
 #include "kernels_mcmc.hpp"
 #include "fuse.hpp"
-#include "lower.hpp"
 #include "tile.hpp"
+#include "tiled_launch.hpp"
 
 namespace plamd {
-
-thread_local std::string g_err;
-
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIP_OK(expr)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (expr);                                                            \
-        if (e_ != hipSuccess) {                                                            \
-            throw DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_));          \
-        }                                                                                  \
-    } while (0)
-
-int device_cus() {
-    static int cus = [] {
-        int dev = 0;
-        hipDeviceProp_t p;
-        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) {
-            return 256;
-        }
-        return p.multiProcessorCount;
-    }();
-    return cus;
-}
-
-// Grid for a grid-stride streaming kernel over `work` items, `per` items per
-// thread per iteration: enough workgroups to fill every CU several times,
-// but no more than the work needs.
-unsigned grid_for(uint64_t work, int per = 1, unsigned cap_per_cu = 32) {
-    const uint64_t need = (work + uint64_t(kBlock) * per - 1) / (uint64_t(kBlock) * per);
-    const uint64_t cap = uint64_t(device_cus()) * cap_per_cu;
-    return static_cast<unsigned>(std::max<uint64_t>(1, std::min(need, cap)));
-}
-
-// ---------------------------------------------------------------------------
-// Observable tree parsed from the program encoding of pl_amd.h
-// ---------------------------------------------------------------------------
-struct ObsNode {
-    int kind = 0; // 0 named, 1 hermitian, 2 tensor, 3 hamiltonian
-    int op = 0;
-    std::vector<int> wires;
-    std::vector<std::complex<double>> mat;
-    std::vector<double> coeffs;
-    std::vector<ObsNode> kids;
-};
-
-struct ObsReader {
-    const int *prog;
-    const double *coeffs;
-    const double *mats;
-    ObsNode factor() {
-        ObsNode o;
-        o.kind = *prog++;
-        o.op = *prog++;
-        const int nw = *prog++;
-        if (nw < 0 || nw > 30) {
-            throw InvalidArg("bad observable wire count");
-        }
-        o.wires.assign(prog, prog + nw);
-        prog += nw;
-        if (o.kind == 1) {
-            if (!mats) {
-                throw InvalidArg("Hermitian observable without matrix data");
-            }
-            const size_t dim = size_t{1} << (2 * nw);
-            for (size_t i = 0; i < dim; i++) {
-                o.mat.emplace_back(mats[2 * i], mats[2 * i + 1]);
-            }
-            mats += 2 * dim;
-        } else if (o.kind == 0) {
-            gate_name(o.op); // range check
-        } else {
-            throw InvalidArg("bad observable factor kind");
-        }
-        return o;
-    }
-    ObsNode term() {
-        const int nf = *prog++;
-        if (nf < 1) {
-            throw InvalidArg("empty observable term");
-        }
-        if (nf == 1) {
-            return factor();
-        }
-        ObsNode t;
-        t.kind = 2;
-        std::vector<int> seen;
-        for (int f = 0; f < nf; f++) {
-            t.kids.push_back(factor());
-            for (int w : t.kids.back().wires) {
-                if (std::find(seen.begin(), seen.end(), w) != seen.end()) {
-                    // TensorProdObsBase ctor, Observables.hpp:238-246
-                    throw InvalidArg("All wires in observables must be disjoint.");
-                }
-                seen.push_back(w);
-            }
-        }
-        return t;
-    }
-    ObsNode one() {
-        const int is_ham = *prog++;
-        const int nt = *prog++;
-        if (!is_ham) {
-            if (nt != 1) {
-                throw InvalidArg("non-Hamiltonian observable must have one term");
-            }
-            return term();
-        }
-        ObsNode h;
-        h.kind = 3;
-        for (int t = 0; t < nt; t++) {
-            h.coeffs.push_back(*coeffs++);
-            h.kids.push_back(term());
-        }
-        return h;
-    }
-};
-
-// A Pauli word: (x-mask, zy-mask, #Y) -- the fast path of expval / var /
-// the adjoint's observable application.
-struct PauliWord {
-    uint64_t x = 0, zy = 0;
-    int ny = 0;
-};
-
-bool as_pauli(const ObsNode &o, int n, PauliWord &pw) {
-    auto letter = [&](const ObsNode &f) {
-        if (f.kind != 0 || f.wires.size() != 1) {
-            return false;
-        }
-        if (f.wires[0] < 0 || f.wires[0] >= n) {
-            throw InvalidArg("wire index out of range");
-        }
-        const uint64_t b = uint64_t{1} << (n - 1 - f.wires[0]);
-        switch (f.op) {
-        case Identity: return true;
-        case PauliX: pw.x |= b; return true;
-        case PauliY: pw.x |= b; pw.zy |= b; pw.ny++; return true;
-        case PauliZ: pw.zy |= b; return true;
-        default: return false;
-        }
-    };
-    pw = PauliWord{};
-    if (o.kind == 0) {
-        return letter(o);
-    }
-    if (o.kind == 2) {
-        for (const auto &k : o.kids) {
-            if (!letter(k)) {
-                return false;
-            }
-        }
-        return true;
-    }
-    return false;
-}
-
-} // namespace plamd
-
-#include "batch_host.hpp"
-#include "mlp_host.hpp"
-
-using namespace plamd;
-
-struct pl_batch {
-    std::unique_ptr<BatchPlanBase> plan;
-};
-struct pl_mlp {
-    std::unique_ptr<plamd::Mlp> net;
-};
-
-struct pl_sv {
-    int n = 0;
-    int dtype = PL_C128;
-    uint64_t len = 0;
-    void *d = nullptr;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<void *> pool;   // scratch states (state-sized)
-    size_t pool_used = 0;
-    double *red = nullptr;      // reduction scratch (partials)
-    size_t red_bytes = 0;
-    void *staging = nullptr;    // device staging for matrices / small tables
-    size_t staging_bytes = 0;
-    std::vector<hipEvent_t> slots; // pl_event_record slots
-
-    size_t state_bytes() const { return len * (dtype == PL_C128 ? 16 : 8); }
-};
-
-namespace plamd {
-
-void *scratch_acquire(pl_sv *h) {
-    if (h->pool_used == h->pool.size()) {
-        void *p = nullptr;
-        HIP_OK(hipMalloc(&p, h->state_bytes()));
-        h->pool.push_back(p);
-    }
-    return h->pool[h->pool_used++];
-}
-struct ScratchScope {
-    pl_sv *h;
-    size_t mark;
-    explicit ScratchScope(pl_sv *hh) : h(hh), mark(hh->pool_used) {}
-    void *get() { return scratch_acquire(h); }
-    ~ScratchScope() { h->pool_used = mark; }
-};
-
-double *reduce_buf(pl_sv *h, size_t bytes) {
-    if (h->red_bytes < bytes) {
-        if (h->red) {
-            HIP_OK(hipStreamSynchronize(h->stream));
-            HIP_OK(hipFree(h->red));
-            h->red = nullptr;
-        }
-        HIP_OK(hipMalloc(reinterpret_cast<void **>(&h->red), bytes));
-        h->red_bytes = bytes;
-    }
-    return h->red;
-}
-
-// Copy host bytes to the handle's device staging area and wait for the copy,
-// so the host buffer may be released as soon as this returns.
-void *stage(pl_sv *h, const void *src, size_t bytes) {
-    if (h->staging_bytes < bytes) {
-        HIP_OK(hipStreamSynchronize(h->stream));
-        if (h->staging) {
-            HIP_OK(hipFree(h->staging));
-            h->staging = nullptr;
-        }
-        const size_t sz = std::max<size_t>(bytes, 1 << 16);
-        HIP_OK(hipMalloc(&h->staging, sz));
-        h->staging_bytes = sz;
-    }
-    HIP_OK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_OK(hipStreamSynchronize(h->stream));
-    return h->staging;
-}
 
 template <class T> struct Engine {
     using V = v2<T>;
@@ -313,9 +62,14 @@ template <class T> struct Engine {
                     }
                 }
                 const uint64_t nelem = nwork << 1;
-                hipLaunchKernelGGL((k_pair_lane<T>), dim3(static_cast<unsigned>(nelem / kBlock + (nelem % kBlock ? 1 : 0))),
-                                   dim3(kBlock), 0, h->stream, sv, nelem, geo, p, op.mat[0], op.mat[1],
-                                   op.mat[2], op.mat[3], op.diag);
+                const dim3 grid(static_cast<unsigned>(nelem / kBlock + (nelem % kBlock ? 1 : 0)));
+                if (g.diag) {
+                    hipLaunchKernelGGL((k_pair_lane<T, true>), grid, dim3(kBlock), 0, h->stream, sv, nelem, geo, p,
+                                       op.mat[0], op.mat[1], op.mat[2], op.mat[3]);
+                } else {
+                    hipLaunchKernelGGL((k_pair_lane<T, false>), grid, dim3(kBlock), 0, h->stream, sv, nelem, geo,
+                                       p, op.mat[0], op.mat[1], op.mat[2], op.mat[3]);
+                }
                 HIP_OK(hipGetLastError());
                 return;
             }
@@ -359,20 +113,32 @@ template <class T> struct Engine {
             to.diag = op.diag;
             const uint64_t ntiles = len() >> (kTileLow + nhi);
             const unsigned grid = static_cast<unsigned>(ntiles);
-            switch (nhi) {
-            case 0: hipLaunchKernelGGL((k_group_tile<T, M, 0>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-            case 1: hipLaunchKernelGGL((k_group_tile<T, M, 1>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-            case 2: hipLaunchKernelGGL((k_group_tile<T, M, 2>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-            case 3: hipLaunchKernelGGL((k_group_tile<T, M, 3>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-            default: hipLaunchKernelGGL((k_group_tile<T, M, 4>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+            if (g.diag) {
+                launch_tile<M, true>(nhi, grid, sv, ntiles, to);
+            } else {
+                launch_tile<M, false>(nhi, grid, sv, ntiles, to);
             }
             HIP_OK(hipGetLastError());
             return;
         }
         const unsigned grid = static_cast<unsigned>((nwork + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL((k_group_flat<T, M>), dim3(grid), dim3(kBlock), 0, h->stream, sv, nwork,
-                           op);
+        if (g.diag) {
+            hipLaunchKernelGGL((k_group_flat<T, M, true>), dim3(grid), dim3(kBlock), 0, h->stream, sv, nwork, op);
+        } else {
+            hipLaunchKernelGGL((k_group_flat<T, M, false>), dim3(grid), dim3(kBlock), 0, h->stream, sv, nwork, op);
+        }
         HIP_OK(hipGetLastError());
+    }
+
+    template <int M, bool DIAG>
+    void launch_tile(int nhi, unsigned grid, V *sv, uint64_t ntiles, const TileOp<T, M> &to) {
+        switch (nhi) {
+        case 0: hipLaunchKernelGGL((k_group_tile<T, M, 0, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        case 1: hipLaunchKernelGGL((k_group_tile<T, M, 1, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        case 2: hipLaunchKernelGGL((k_group_tile<T, M, 2, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        case 3: hipLaunchKernelGGL((k_group_tile<T, M, 3, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        default: hipLaunchKernelGGL((k_group_tile<T, M, 4, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        }
     }
 
     template <int M> void launch_big(const GroupOp<T> &g, V *sv) {
@@ -646,15 +412,6 @@ template <class T> struct Engine {
     static constexpr int kTileDefaultBits = std::is_same_v<T, double> ? 10 : 11;
     static constexpr int kTileDefaultLow = std::is_same_v<T, double> ? 3 : 4;
 
-    // one wavefront per tile; LDS only for passes with lds ops
-    template <int KB>
-    void launch_tile_pass(V *sv, uint64_t ntiles, const TpGeom &g, const TrInst<T> *prog, int ninst,
-                          const TpOp<T> *lops, bool uses_lds) {
-        const size_t lds = uses_lds ? (size_t{1} << (6 + KB)) * sizeof(V) : 0;
-        hipLaunchKernelGGL((k_tile_pass<T, KB>), dim3(static_cast<unsigned>(ntiles)), dim3(kTpLanes), lds,
-                           h->stream, sv, ntiles, g, prog, ninst, lops);
-        HIP_OK(hipGetLastError());
-    }
 
     // Applies a tape as tile passes (tile.hpp); returns the number of launches.
     // k = 0 / low < 0 pick the measured defaults of the precision; with k = 0
@@ -755,12 +512,7 @@ template <class T> struct Engine {
             const TrInst<T> *pp = dprog + range[p].first;
             const int ni = range[p].second;
             const bool ul = uses_lds[p] != 0;
-            switch (k - 6) {
-            case 3: launch_tile_pass<3>(sv, ntiles, g, pp, ni, dlops, ul); break;
-            case 4: launch_tile_pass<4>(sv, ntiles, g, pp, ni, dlops, ul); break;
-            case 5: launch_tile_pass<5>(sv, ntiles, g, pp, ni, dlops, ul); break;
-            default: launch_tile_pass<6>(sv, ntiles, g, pp, ni, dlops, ul); break;
-            }
+            HIP_OK(launch_tile_pass<T>(k - 6, sv, ntiles, g, pp, ni, dlops, ul, h->stream));
         }
         HIP_OK(hipStreamSynchronize(h->stream));
         return static_cast<int>(passes.size());
@@ -796,52 +548,64 @@ template <class T> struct Engine {
     // greedily into passes whose high bits fit one LDS tile (k_pauli_tile);
     // each pass reads the state once.  Masks wider than a tile get a pass of
     // their own (k_pauli_expval).
+    // ext / ext_bytes: a caller-provided device area for the term tables and
+    // partial sums (nothing is allocated then); nullptr: the handle's buffers.
     double pauli_expval(const V *sv, const std::vector<PauliWord> &words,
-                        const std::vector<double> &coeffs) {
+                        const std::vector<double> &coeffs, unsigned char *ext = nullptr,
+                        size_t ext_bytes = 0) {
         std::map<uint64_t, std::vector<PauliTerm>> groups;
         for (size_t t = 0; t < words.size(); t++) {
             groups[words[t].x].push_back(PauliTerm{words[t].zy, coeffs[t], words[t].ny & 3, 0});
         }
         const int nlow = std::min(3, n());
         const uint64_t lowm = (uint64_t{1} << nlow) - 1;
-        std::vector<uint64_t> masks;
+        // work items: an x-mask with at most kPauliMaxTerms of its terms
+        struct Item {
+            uint64_t x;
+            std::vector<PauliTerm> terms;
+        };
+        std::vector<Item> items;
+        std::vector<uint64_t> wide;
         for (auto &kv : groups) {
-            if (kv.first) {
-                masks.push_back(kv.first);
+            if (kv.first && __builtin_popcountll(kv.first & ~lowm) > kPauliTileHigh) {
+                wide.push_back(kv.first);
+                continue;
+            }
+            for (size_t a = 0; a < kv.second.size(); a += kPauliMaxTerms) {
+                const size_t b = std::min(kv.second.size(), a + kPauliMaxTerms);
+                items.push_back(Item{kv.first, {kv.second.begin() + a, kv.second.begin() + b}});
             }
         }
-        std::stable_sort(masks.begin(), masks.end(), [&](uint64_t a, uint64_t b) {
-            return __builtin_popcountll(a & ~lowm) > __builtin_popcountll(b & ~lowm);
+        // widest high parts first; the diagonal items (x = 0) fit any pass
+        std::stable_sort(items.begin(), items.end(), [&](const Item &a, const Item &b) {
+            return __builtin_popcountll(a.x & ~lowm) > __builtin_popcountll(b.x & ~lowm);
         });
         struct Pass {
             uint64_t high = 0;
-            std::vector<uint64_t> xs;
+            size_t nterms = 0;
+            std::vector<const Item *> its;
         };
         std::vector<Pass> passes;
-        std::vector<uint64_t> wide;
-        std::vector<uint64_t> rest = masks;
+        std::vector<const Item *> rest;
+        for (const auto &it : items) {
+            rest.push_back(&it);
+        }
         while (!rest.empty()) {
             Pass ps;
-            std::vector<uint64_t> left;
-            for (uint64_t x : rest) {
-                const uint64_t xh = x & ~lowm;
-                if (__builtin_popcountll(xh) > kPauliTileHigh) {
-                    wide.push_back(x);
-                } else if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh) {
+            std::vector<const Item *> left;
+            for (const Item *it : rest) {
+                const uint64_t xh = it->x & ~lowm;
+                if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh &&
+                    ps.nterms + it->terms.size() <= static_cast<size_t>(kPauliMaxTerms)) {
                     ps.high |= xh;
-                    ps.xs.push_back(x);
+                    ps.nterms += it->terms.size();
+                    ps.its.push_back(it);
                 } else {
-                    left.push_back(x);
+                    left.push_back(it);
                 }
             }
-            if (!ps.xs.empty()) {
-                passes.push_back(std::move(ps));
-            }
+            passes.push_back(std::move(ps));
             rest.swap(left);
-        }
-        const bool diag = groups.count(0) > 0;
-        if (diag && passes.empty()) {
-            passes.emplace_back(); // the diagonal terms alone: one pass over the state
         }
         // per pass: geometry, groups and compressed terms, all staged together
         std::vector<PauliTileGeom> geos;
@@ -872,13 +636,10 @@ template <class T> struct Engine {
             };
             geos.push_back(g);
             const size_t g0 = gtab.size(), t0 = ttab.size();
-            std::vector<uint64_t> xs = ps.xs;
-            if (pi == 0 && diag) {
-                xs.insert(xs.begin(), 0);
-            }
-            for (uint64_t x : xs) {
+            for (const Item *it : ps.its) {
+                const uint64_t x = it->x;
                 PauliTileGroup gr{compress(x), static_cast<int>(ttab.size()), 0};
-                for (const auto &pt : groups[x]) {
+                for (const auto &pt : it->terms) {
                     const uint32_t zyl = compress(pt.zy & smask);
                     double cr = pt.coeff, ci = 0;
                     if (x != 0) { // see PauliTileTerm
@@ -910,12 +671,26 @@ template <class T> struct Engine {
         std::memcpy(blob.data(), gtab.data(), gb);
         std::memcpy(blob.data() + tb_off, ttab.data(), tb);
         std::memcpy(blob.data() + wb_off, wtab.data(), wtab.size() * sizeof(PauliTerm));
-        auto *dev = static_cast<unsigned char *>(stage(h, blob.data(), blob.size()));
+        const size_t nlaunch = passes.size() + wide.size();
+        const size_t part_bytes = (nlaunch * kReduceBlocks + 1) * sizeof(double2);
+        unsigned char *dev = nullptr;
+        double2 *part = nullptr;
+        if (ext) {
+            const size_t part_off = (blob.size() + 255) & ~size_t{255};
+            if (part_off + part_bytes > ext_bytes) {
+                throw InvalidArg("the observable needs " + std::to_string(part_off + part_bytes) +
+                                 " bytes of scratch; the allocation-free path has " + std::to_string(ext_bytes));
+            }
+            HIP_OK(hipMemcpyAsync(ext, blob.data(), blob.size(), hipMemcpyHostToDevice, h->stream));
+            dev = ext;
+            part = reinterpret_cast<double2 *>(ext + part_off);
+        } else {
+            dev = static_cast<unsigned char *>(stage(h, blob.data(), blob.size()));
+            part = reinterpret_cast<double2 *>(reduce_buf(h, part_bytes));
+        }
         const auto *dgroups = reinterpret_cast<const PauliTileGroup *>(dev);
         const auto *dtterms = reinterpret_cast<const PauliTileTerm *>(dev + tb_off);
         const auto *dwterms = reinterpret_cast<const PauliTerm *>(dev + wb_off);
-        const size_t nlaunch = passes.size() + wide.size();
-        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (nlaunch * kReduceBlocks + 1) * sizeof(double2)));
         int total = 0;
         for (size_t pi = 0; pi < passes.size(); pi++) {
             const PauliTileGeom &g = geos[pi];
@@ -924,7 +699,8 @@ template <class T> struct Engine {
             const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(ntiles, kReduceBlocks));
             hipLaunchKernelGGL((k_pauli_tile<T>), dim3(grid), dim3(kBlock), (size_t{1} << nS) * sizeof(V),
                                h->stream, sv, ntiles, g, static_cast<int>(gspan[pi].second),
-                               dgroups + gspan[pi].first, dtterms, part + total);
+                               dgroups + gspan[pi].first, dtterms, static_cast<int>(tspan[pi].first),
+                               static_cast<int>(tspan[pi].second), part + total);
             HIP_OK(hipGetLastError());
             total += grid;
         }
@@ -1118,31 +894,47 @@ template <class T> struct Engine {
         HIP_OK(hipStreamSynchronize(h->stream));
     }
 
-    // Inverse-CDF sampling (kernels_sample.hpp).  Either host uniforms
-    // (-> indices) or a SplitMix64 seed (-> bit rows).
+    // Exclusive-prefix tables of |psi|^2 for inverse-CDF sampling
+    // (kernels_sample.hpp), in the handle's buffers, for the current state.
+    struct CdfTables {
+        const double *gpre, *tpre;
+        uint64_t ngroups, ntiles;
+    };
+    CdfTables cdf_tables() {
+        const uint64_t ngroups = (len() + kCdfGroup - 1) / kCdfGroup;
+        const uint64_t ntiles = (ngroups + kCdfTile - 1) / kCdfTile;
+        auto *gsum = static_cast<double *>(buf(h, kBufCdfGroups, ngroups * sizeof(double)));
+        auto *gpre = static_cast<double *>(buf(h, kBufCdfPrefix, ngroups * sizeof(double)));
+        auto *tsum = static_cast<double *>(buf(h, kBufCdfTiles, (2 * ntiles + 1) * sizeof(double)));
+        double *tpre = tsum + ntiles; // ntiles + 1 entries, the last one the total
+        const uint64_t nwaves = (ngroups + 4 * kCdfSteps - 1) / (4 * kCdfSteps);
+        hipLaunchKernelGGL((k_cdf_groups<T>), dim3(static_cast<unsigned>((nwaves + 3) / 4)), dim3(kBlock), 0,
+                           h->stream, psi(), len(), ngroups, gsum);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_cdf_tile_sums, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0, h->stream,
+                           gsum, ngroups, tsum);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, h->stream, tsum, ntiles, tpre);
+        HIP_OK(hipGetLastError());
+        hipLaunchKernelGGL(k_cdf_tile_scan, dim3(static_cast<unsigned>(ntiles)), dim3(kBlock), 0, h->stream,
+                           gsum, ngroups, tpre, gpre);
+        HIP_OK(hipGetLastError());
+        return {gpre, tpre, ngroups, ntiles};
+    }
+    // didx[s] for the uniforms du[s], both in device memory (queued, no wait)
+    void sample_device(const double *du, int64_t m, int64_t *didx) {
+        const CdfTables c = cdf_tables();
+        hipLaunchKernelGGL((k_cdf_search<T>), dim3(static_cast<unsigned>((m + kBlock - 1) / kBlock)), dim3(kBlock),
+                           0, h->stream, psi(), len(), c.ngroups, c.ntiles, c.tpre, c.gpre, du, m, didx);
+        HIP_OK(hipGetLastError());
+    }
+
+    // Inverse-CDF sampling.  Either host uniforms (-> indices) or a
+    // SplitMix64 seed (-> bit rows).
     void sample(const double *host_u, uint64_t seed, int64_t m, int64_t *host_idx,
                 uint64_t *host_bits) {
-        const int sb = static_cast<int>(std::min<uint64_t>(kSampleBlock, len()));
-        const uint64_t nsb = (len() + sb - 1) / sb;
-        double *du = nullptr, *bsum = nullptr, *prefix = nullptr;
-        int64_t *didx = nullptr;
-        uint64_t *dbits = nullptr;
-        struct Free {
-            std::vector<void *> p;
-            ~Free() {
-                for (void *q : p) {
-                    (void)hipFree(q);
-                }
-            }
-        } fr;
-        HIP_OK(hipMalloc(&du, m * sizeof(double)));
-        fr.p.push_back(du);
-        HIP_OK(hipMalloc(&didx, m * sizeof(int64_t)));
-        fr.p.push_back(didx);
-        HIP_OK(hipMalloc(&bsum, nsb * sizeof(double)));
-        fr.p.push_back(bsum);
-        HIP_OK(hipMalloc(&prefix, (nsb + 1) * sizeof(double)));
-        fr.p.push_back(prefix);
+        auto *du = static_cast<double *>(buf(h, kBufUniforms, m * sizeof(double)));
+        auto *didx = static_cast<int64_t *>(buf(h, kBufIndices, m * sizeof(int64_t)));
         if (host_u) {
             HIP_OK(hipMemcpyAsync(du, host_u, m * sizeof(double), hipMemcpyHostToDevice, h->stream));
         } else {
@@ -1150,23 +942,14 @@ template <class T> struct Engine {
                                h->stream, seed, m, du);
             HIP_OK(hipGetLastError());
         }
-        const unsigned waves_per_block = kBlock / 64;
-        hipLaunchKernelGGL((k_sample_blocksum<T>), dim3((nsb + waves_per_block - 1) / waves_per_block),
-                           dim3(kBlock), 0, h->stream, psi(), len(), sb, nsb, bsum);
-        HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, h->stream, bsum, nsb, prefix);
-        HIP_OK(hipGetLastError());
-        hipLaunchKernelGGL((k_sample_search<T>), dim3((m + waves_per_block - 1) / waves_per_block),
-                           dim3(kBlock), 0, h->stream, psi(), len(), sb, nsb, prefix, du, m, didx);
-        HIP_OK(hipGetLastError());
+        sample_device(du, m, didx);
         if (host_idx) {
             HIP_OK(hipMemcpyAsync(host_idx, didx, m * sizeof(int64_t), hipMemcpyDeviceToHost,
                                   h->stream));
         }
         if (host_bits) {
-            HIP_OK(hipMalloc(&dbits, m * n() * sizeof(uint64_t)));
-            fr.p.push_back(dbits);
             const uint64_t tot = uint64_t(m) * n();
+            auto *dbits = static_cast<uint64_t *>(buf(h, kBufBits, tot * sizeof(uint64_t)));
             hipLaunchKernelGGL(k_sample_bits, dim3((tot + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                h->stream, didx, m, n(), dbits);
             HIP_OK(hipGetLastError());
@@ -1227,9 +1010,17 @@ template <class T> struct Engine {
         const uint64_t chains = nchains > 0 ? std::min<uint64_t>(nchains, m)
                                             : std::min<uint64_t>(m, uint64_t{16384});
         const uint64_t per = (m + chains - 1) / chains;
+        // every chain starts from an inverse-CDF draw, i.e. in the stationary
+        // distribution |psi|^2, so that short chains are still unbiased
+        auto *ustart = static_cast<double *>(buf(h, kBufUniforms, chains * sizeof(double)));
+        auto *start = static_cast<int64_t *>(buf(h, kBufChainStart, chains * sizeof(int64_t)));
+        hipLaunchKernelGGL(k_uniforms, dim3(static_cast<unsigned>((chains + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           h->stream, seed ^ 0xA0761D6478BD642Full, static_cast<int64_t>(chains), ustart);
+        HIP_OK(hipGetLastError());
+        sample_device(ustart, static_cast<int64_t>(chains), start);
         auto *didx = static_cast<int64_t *>(alloc(m * sizeof(int64_t)));
         hipLaunchKernelGGL((k_mcmc_chains<T>), dim3(static_cast<unsigned>((chains + kBlock - 1) / kBlock)),
-                           dim3(kBlock), 0, h->stream, psi(), len(), n(), kind, nz, nnz,
+                           dim3(kBlock), 0, h->stream, psi(), len(), n(), kind, nz, nnz, start,
                            static_cast<uint64_t>(burnin), per, static_cast<uint64_t>(m), seed, didx);
         HIP_OK(hipGetLastError());
         const uint64_t tot = uint64_t(m) * n();
@@ -1242,21 +1033,33 @@ template <class T> struct Engine {
     }
 
     // Measurements::expval(Observable) and var(Observable)
-    double expval(const ObsNode &o, bool var) {
+    // a Pauli word or a Hamiltonian of Pauli words as (words, coefficients)
+    bool as_pauli_sum(const ObsNode &o, std::vector<PauliWord> &ws, std::vector<double> &cs) {
         PauliWord pw;
-        if (!var) {
-            if (as_pauli(o, n(), pw)) {
-                return pauli_expval(psi(), {pw}, {1.0});
+        if (as_pauli(o, n(), pw)) {
+            ws = {pw};
+            cs = {1.0};
+            return true;
+        }
+        if (o.kind != 3 || o.kids.empty()) {
+            return false;
+        }
+        ws.assign(o.kids.size(), PauliWord{});
+        for (size_t t = 0; t < o.kids.size(); t++) {
+            if (!as_pauli(o.kids[t], n(), ws[t])) {
+                return false;
             }
-            if (o.kind == 3) {
-                std::vector<PauliWord> ws(o.kids.size());
-                bool all = !o.kids.empty();
-                for (size_t t = 0; t < o.kids.size() && all; t++) {
-                    all = as_pauli(o.kids[t], n(), ws[t]);
-                }
-                if (all) {
-                    return pauli_expval(psi(), ws, o.coeffs);
-                }
+        }
+        cs = o.coeffs;
+        return true;
+    }
+
+    double expval(const ObsNode &o, bool var) {
+        if (!var) {
+            std::vector<PauliWord> ws;
+            std::vector<double> cs;
+            if (as_pauli_sum(o, ws, cs)) {
+                return pauli_expval(psi(), ws, cs);
             }
         }
         ScratchScope sc(h);
@@ -1270,25 +1073,6 @@ template <class T> struct Engine {
         return ms - e * e;
     }
 };
-
-template <class F> int guarded(F &&f) {
-    try {
-        f();
-        return PL_OK;
-    } catch (const OutOfRange &e) {
-        g_err = e.what();
-        return PL_ERR_RANGE;
-    } catch (const InvalidArg &e) {
-        g_err = e.what();
-        return PL_ERR_INVALID;
-    } catch (const DeviceError &e) {
-        g_err = e.what();
-        return PL_ERR_DEVICE;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return PL_ERR_INVALID;
-    }
-}
 
 // Engine of the handle's precision, exceptions propagate
 template <class F> void dispatch_or_throw(pl_sv_t h, F &&f) {
@@ -1584,6 +1368,8 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
 
 } // namespace plamd
 
+using namespace plamd;
+
 // ===========================================================================
 // extern "C"
 // ===========================================================================
@@ -1712,6 +1498,7 @@ int pl_sv_create(int num_qubits, int dtype, pl_sv_t *out) {
         HIP_OK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&h->ev0));
         HIP_OK(hipEventCreate(&h->ev1));
+        HIP_OK(hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming));
         hipError_t e = hipMalloc(&h->d, h->state_bytes());
         if (e != hipSuccess) {
             (void)hipStreamDestroy(h->stream);
@@ -1738,6 +1525,9 @@ int pl_sv_destroy(pl_sv_t h) {
         }
         (void)hipFree(h->red);
         (void)hipFree(h->staging);
+        for (auto &b : h->bufs) {
+            (void)hipFree(b.first);
+        }
         (void)hipFree(h->d);
         for (hipEvent_t e : h->slots) {
             if (e) {
@@ -1746,6 +1536,9 @@ int pl_sv_destroy(pl_sv_t h) {
         }
         (void)hipEventDestroy(h->ev0);
         (void)hipEventDestroy(h->ev1);
+        if (h->ev_order) {
+            (void)hipEventDestroy(h->ev_order);
+        }
         (void)hipStreamDestroy(h->stream);
         delete h;
     });
@@ -1772,6 +1565,7 @@ int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {
         }
         HIP_OK(hipMemcpyAsync(h->d, host, nbytes, hipMemcpyHostToDevice, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
+        g_h2d_bytes.fetch_add(nbytes, std::memory_order_relaxed);
     });
 }
 int pl_sv_download(pl_sv_t h, void *host, size_t nbytes) {
@@ -1782,6 +1576,13 @@ int pl_sv_download(pl_sv_t h, void *host, size_t nbytes) {
         }
         HIP_OK(hipMemcpyAsync(host, h->d, nbytes, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
+        g_d2h_bytes.fetch_add(nbytes, std::memory_order_relaxed);
+    });
+}
+int pl_transfer_counters(uint64_t *h2d_bytes, uint64_t *d2h_bytes) {
+    return guarded([&] {
+        if (h2d_bytes) *h2d_bytes = g_h2d_bytes.load(std::memory_order_relaxed);
+        if (d2h_bytes) *d2h_bytes = g_d2h_bytes.load(std::memory_order_relaxed);
     });
 }
 int pl_sv_copy(pl_sv_t dst, pl_sv_t src) {
@@ -1790,8 +1591,17 @@ int pl_sv_copy(pl_sv_t dst, pl_sv_t src) {
         if (dst->n != src->n || dst->dtype != src->dtype) {
             throw InvalidArg("state vectors differ in size or precision");
         }
-        HIP_OK(hipStreamSynchronize(src->stream));
+        if (dst == src) {
+            return;
+        }
+        // dst's stream waits for src's queued work, copies, and src's stream
+        // then waits for the copy, so later work queued on src (a gate) cannot
+        // overwrite amplitudes the copy has not read yet
+        HIP_OK(hipEventRecord(src->ev_order, src->stream));
+        HIP_OK(hipStreamWaitEvent(dst->stream, src->ev_order, 0));
         HIP_OK(hipMemcpyAsync(dst->d, src->d, dst->state_bytes(), hipMemcpyDeviceToDevice, dst->stream));
+        HIP_OK(hipEventRecord(dst->ev_order, dst->stream));
+        HIP_OK(hipStreamWaitEvent(src->stream, dst->ev_order, 0));
     });
 }
 int pl_sv_set_basis_state(pl_sv_t h, uint64_t index) {
@@ -2192,169 +2002,6 @@ int pl_generate_mcmc_samples(pl_sv_t h, int kernel, int64_t num_burnin, int64_t 
             throw InvalidArg("null output");
         }
         E.mcmc(kernel, num_burnin, nshots, seed, nchains, bits);
-    });
-}
-
-int pl_batch_create(int dtype, int ncirc, const int *nqubits, const int *depth, const char *pattern,
-                    int entangler, const int *prog, const double *coeffs, int tile_width,
-                    pl_batch_t *out) {
-    return guarded([&] {
-        if (!out || !nqubits || !depth || !pattern || !prog) {
-            throw InvalidArg("null argument");
-        }
-        if (ncirc < 1) {
-            throw InvalidArg("need at least one circuit");
-        }
-        if (dtype != PL_C64 && dtype != PL_C128) {
-            throw InvalidArg("dtype must be PL_C64 or PL_C128");
-        }
-        HIP_OK(hipSetDevice(0));
-        ObsReader rd{prog, coeffs, nullptr};
-        if (*rd.prog++ != ncirc) {
-            throw InvalidArg("need one observable per circuit");
-        }
-        std::vector<std::vector<PauliTerm>> terms(ncirc);
-        std::vector<std::vector<uint64_t>> xm(ncirc);
-        for (int j = 0; j < ncirc; j++) {
-            const ObsNode o = rd.one();
-            std::vector<const ObsNode *> parts;
-            std::vector<double> cs;
-            if (o.kind == 3) {
-                for (size_t t = 0; t < o.kids.size(); t++) {
-                    parts.push_back(&o.kids[t]);
-                    cs.push_back(o.coeffs[t]);
-                }
-            } else {
-                parts.push_back(&o);
-                cs.push_back(1.0);
-            }
-            for (size_t t = 0; t < parts.size(); t++) {
-                PauliWord pw;
-                if (!as_pauli(*parts[t], nqubits[j], pw)) {
-                    throw InvalidArg("batched observables must be sums of Pauli words");
-                }
-                terms[j].push_back(PauliTerm{pw.zy, cs[t], pw.ny & 3, 0});
-                xm[j].push_back(pw.x);
-            }
-        }
-        auto b = std::make_unique<pl_batch>();
-        if (dtype == PL_C128) {
-            b->plan = std::make_unique<BatchPlan<double>>(ncirc, nqubits, depth, pattern, entangler,
-                                                          terms, xm, tile_width);
-        } else {
-            b->plan = std::make_unique<BatchPlan<float>>(ncirc, nqubits, depth, pattern, entangler,
-                                                         terms, xm, tile_width);
-        }
-        *out = b.release();
-    });
-}
-
-int pl_batch_run(pl_batch_t b, const double *params, double *energies, double *grads) {
-    return guarded([&] {
-        if (!b || !energies || (!params && b->plan->total_params() > 0)) {
-            throw InvalidArg("null argument");
-        }
-        HIP_OK(hipSetDevice(0));
-        b->plan->run(params, energies, grads);
-    });
-}
-
-int pl_batch_info(pl_batch_t b, int *tile_width, int *launches, int64_t *nparams) {
-    return guarded([&] {
-        if (!b) throw InvalidArg("null argument");
-        if (tile_width) *tile_width = b->plan->tile_width();
-        if (launches) *launches = b->plan->launches();
-        if (nparams) *nparams = static_cast<int64_t>(b->plan->total_params());
-    });
-}
-
-int pl_batch_destroy(pl_batch_t b) {
-    return guarded([&] { delete b; });
-}
-
-int pl_mlp_create(int d0, int d1, int d2, int d3, int max_batch, uint64_t seed, pl_mlp_t *out) {
-    return guarded([&] {
-        if (!out || d0 < 1 || d1 < 1 || d2 < 1 || d3 < 1 || max_batch < 1 || max_batch > (1 << 20)) {
-            throw InvalidArg("invalid MLP shape");
-        }
-        HIP_OK(hipSetDevice(0));
-        auto m = std::make_unique<pl_mlp>();
-        m->net = std::make_unique<Mlp>(d0, d1, d2, d3, max_batch, seed);
-        *out = m.release();
-    });
-}
-
-int pl_mlp_destroy(pl_mlp_t m) {
-    return guarded([&] { delete m; });
-}
-
-static Mlp &mlp_of(pl_mlp_t m) {
-    if (!m) {
-        throw InvalidArg("null MLP handle");
-    }
-    HIP_OK(hipSetDevice(0));
-    return *m->net;
-}
-
-int pl_mlp_num_params(pl_mlp_t m, int64_t *out) {
-    return guarded([&] {
-        if (!out) throw InvalidArg("null output");
-        *out = static_cast<int64_t>(mlp_of(m).nparam);
-    });
-}
-
-int pl_mlp_get_params(pl_mlp_t m, float *params) {
-    return guarded([&] {
-        Mlp &net = mlp_of(m);
-        if (!params) throw InvalidArg("null output");
-        HIP_OK(hipStreamSynchronize(net.stream));
-        HIP_OK(hipMemcpy(params, net.p32, net.nparam * sizeof(float), hipMemcpyDeviceToHost));
-    });
-}
-
-int pl_mlp_set_params(pl_mlp_t m, const float *params) {
-    return guarded([&] {
-        if (!params) throw InvalidArg("null input");
-        mlp_of(m).set_params(params);
-    });
-}
-
-int pl_mlp_set_loss_weights(pl_mlp_t m, const float *w) {
-    return guarded([&] {
-        Mlp &net = mlp_of(m);
-        if (!w) throw InvalidArg("null input");
-        HIP_OK(hipStreamSynchronize(net.stream));
-        HIP_OK(hipMemcpy(net.colw, w, net.d[3] * sizeof(float), hipMemcpyHostToDevice));
-    });
-}
-
-int pl_mlp_predict(pl_mlp_t m, const float *x, int n, float *y) {
-    return guarded([&] {
-        if (!x || !y) throw InvalidArg("null argument");
-        mlp_of(m).predict(x, n, y);
-    });
-}
-
-int pl_mlp_gradients(pl_mlp_t m, const float *x, const float *t, int n, double *loss, float *grads) {
-    return guarded([&] {
-        if (!x || !t || !grads) throw InvalidArg("null argument");
-        const double l = mlp_of(m).gradients(x, t, n, grads);
-        if (loss) *loss = l;
-    });
-}
-
-int pl_mlp_load_data(pl_mlp_t m, const float *x, const float *t, int64_t n) {
-    return guarded([&] {
-        if (!x || !t || n < 1) throw InvalidArg("invalid training set");
-        mlp_of(m).load_data(x, t, n);
-    });
-}
-
-int pl_mlp_train(pl_mlp_t m, int64_t steps, int batch, float lr, double *mean_loss) {
-    return guarded([&] {
-        if (steps < 0) throw InvalidArg("negative step count");
-        const double l = mlp_of(m).train(steps, batch, lr);
-        if (mean_loss) *mean_loss = l;
     });
 }
 

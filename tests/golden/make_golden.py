@@ -151,8 +151,80 @@ def random_circuit_small(ref):
                         expval=np.array([e]), norm=np.array([np.linalg.norm(s)]))
 
 
+def cdf_indices_exact(s, u, chunk=1 << 24):
+    """Inverse-CDF indices of the uniforms `u` (any order) on state `s`, with
+    the prefix sums of |psi|^2 carried in extended precision so that the
+    result does not depend on a summation order: index i with
+    C_i < u <= C_{i+1}, C exclusive, index 0 for u <= C_1 and the last index
+    for u above the total (the rule of MeasuresFunctors.hpp:81-98, where
+    C_N := 1).  Also returns each uniform's distance to the nearer of its two
+    boundaries, which tells a comparison where two summation orders may
+    legitimately disagree."""
+    order = np.argsort(u, kind="stable")
+    us = u[order].astype(np.longdouble)
+    idx = np.full(u.size, s.size - 1, np.int64)
+    gap = np.full(u.size, np.inf)
+    base = np.longdouble(0)
+    j = 0
+    for lo in range(0, s.size, chunk):
+        a = s[lo:lo + chunk]
+        p = (a.real.astype(np.float64) ** 2 + a.imag.astype(np.float64) ** 2).astype(np.longdouble)
+        incl = base + np.cumsum(p)
+        hi_j = int(np.searchsorted(us, incl[-1], side="right"))
+        if hi_j > j:
+            k = np.searchsorted(incl, us[j:hi_j], side="left")
+            idx[order[j:hi_j]] = lo + k
+            upper = incl[k]
+            lower = np.where(k > 0, incl[np.maximum(k - 1, 0)], base)
+            gap[order[j:hi_j]] = np.minimum(upper - us[j:hi_j], us[j:hi_j] - lower).astype(np.float64)
+            j = hi_j
+        base = incl[-1]
+    return idx, gap
+
+
+def c2_full(ref, n=30):
+    """The single-large-state workload at its full size: the seeded 50-layer
+    gate mix on 30 qubits (complex128, 16 GiB) run by the reference's own
+    kernels, then the 100-term Hamiltonian term by term through the
+    reference's Measurements::expval (copy, apply, innerProdC: 32 GiB peak),
+    amplitudes at 4096 seeded indices and the inverse-CDF indices of the
+    first 10^4 of the benchmark's 10^6 uniforms.  Scalars and small arrays
+    only are written; the state is never stored."""
+    import time
+    from lightning_unification_amd.circuits import sample_uniforms
+    tape = random_layers(n, 50, seed=1002)
+    ham = random_hamiltonian(n, 100, seed=1002)
+    s = np.zeros(1 << n, np.complex128)
+    s[0] = 1
+    t0 = time.time()
+    for k in range(0, len(tape), 10):
+        part = Tape()
+        part.ops = tape.ops[k:k + 10]
+        ref.apply_tape(s, part)
+        print(f"forward {k + len(part)}/{len(tape)} gates, {time.time() - t0:.0f} s", flush=True)
+    idx = np.sort(np.random.default_rng(30).choice(1 << n, 4096, replace=False))
+    amps = s[idx].copy()
+    norm = float(np.sqrt(np.vdot(s, s).real))
+    u = sample_uniforms(10 ** 6, seed=1002)[:10 ** 4]
+    sidx, gap = cdf_indices_exact(s, u)
+    terms = []
+    for t, term in enumerate(ham.terms):
+        terms.append(ref.expval(s, term))
+        print(f"term {t + 1}/{len(ham.terms)}: {terms[-1]:.17g}, {time.time() - t0:.0f} s", flush=True)
+    terms = np.array(terms)
+    coeffs = np.array(ham.coeffs)
+    np.savez_compressed(os.path.join(HERE, f"c2_n{n}.npz"), idx=idx, amps=amps,
+                        norm=np.array([norm]), term_expvals=terms, coeffs=coeffs,
+                        expval=np.array([float(np.dot(coeffs, terms))]),
+                        uniforms=u, sample_idx=sidx, sample_gap=gap)
+
+
 if __name__ == "__main__":
     ref = Oracle(ref=True)
+    if len(sys.argv) > 1 and sys.argv[1] == "c2_n30":
+        c2_full(ref, 30)
+        print("c2_n30 written to", HERE)
+        sys.exit(0)
     gates(ref)
     measurements(ref)
     adjoint(ref)

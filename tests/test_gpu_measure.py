@@ -93,7 +93,7 @@ def boundary_ties(cdf_incl, u, idx_a, idx_b, band=1e-12):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [1, 6, 14])
+@pytest.mark.parametrize("n", [1, 6, 14, 20, 23])
 def test_sampling_bit_exact(oracle, dtype, n):
     capi = _capi()
     s = random_state(n, 5 + n, dtype)

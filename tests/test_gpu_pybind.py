@@ -19,13 +19,13 @@ def ops():
 
 
 def apply_lightning(ops, state, tape, c64=False):
-    """lightning_qubit.py:330-369: one StateVectorC* view, one call per gate."""
+    """lightning_qubit.py:330-369: one StateVectorC* view, one call per gate;
+    the wrapper is released with this frame, as in the device."""
     state_vector = np.ravel(state)
     sim = ops.StateVectorC64(state_vector) if c64 else ops.StateVectorC128(state_vector)
     for name, wires, inv, params in tape.ops:
         method = getattr(sim, name, None)
         method(wires, inv, params)
-    del sim  # the wrapper writes the device amplitudes back when released
     return np.reshape(state_vector, state.shape)
 
 

@@ -85,6 +85,25 @@ def test_mcmc_distribution(kernel):
     sv.close()
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_mcmc_peaked_product_state_short_chains(kernel):
+    """22 qubits, every qubit |1> with probability 0.9, the default burn-in
+    of 100 steps and 10^5 shots (about six recorded steps per chain): each
+    wire's marginal must still be 0.9.  Chains started at index 0 would leave
+    about a fifth of the bits never flipped by local proposals."""
+    from lightning_unification_amd import capi
+    n = 22
+    theta = 2 * np.arcsin(np.sqrt(0.9))
+    sv = capi.DeviceStateVector(n)
+    for w in range(n):
+        sv.apply("RY", [w], params=[theta])
+    print(f"mcmc peaked kernel {kernel}", flush=True)
+    a = sv.generate_mcmc_samples(100000, kernel=kernel, num_burnin=100, seed=11)
+    marg = a.mean(axis=0)
+    assert np.all(np.abs(marg - 0.9) < 0.01), marg
+    sv.close()
+
+
 def test_mcmc_sparse_support():
     """Proposals uniform over the non-zero amplitudes reach a state whose
     support excludes index 0 and never leave the support."""
