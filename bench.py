@@ -32,19 +32,23 @@ from lightning_unification_amd.gates import touched_fraction  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+DIAGONAL = {"RZ", "CRZ", "IsingZZ"}  # lowered with a diagonal matrix (the DIAG instantiations)
+
+
 def kernel_for(name, wires, n):
     """Kernel instantiation a gate of this workload runs as (the dispatch of
     Engine::launch_small in csrc/pl_amd.hip)."""
     bits = [n - 1 - w for w in wires]
     m = 1 if name == "T" else (4 if name in ("IsingXX", "IsingZZ") else 2)
+    diag = "true" if name in DIAGONAL else "false"
     if m == 2 and name != "SWAP":
         target, others = bits[-1], bits[:-1]  # 1q gates, CRX/CRZ, Toffoli
         if target < 6 and all(b >= 6 for b in others) and n - len(others) >= 6:
-            return "k_pair_lane<double>"
+            return f"k_pair_lane<double,{diag}>"
     nhi = sum(1 for b in bits if b >= 8)
-    if min(bits) < 3 and nhi <= 4 and n >= 8 + nhi:
-        return f"k_group_tile<double,{m}>"
-    return f"k_group_flat<double,{m}>"
+    if min(bits) < 3 and nhi <= 3 and n >= 8 + nhi:
+        return f"k_group_tile<double,{m},{nhi},{diag}>"
+    return f"k_group_flat<double,{m},{diag}>"
 
 
 def log(msg):
@@ -94,7 +98,32 @@ def cpu_baseline(tape, n, budget_s=20.0):
     del sv
     return {"value": round(nbytes / t_total / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": kind,
             "sample": f"first {done} gates of the same {n}-qubit complex128 circuit, "
-                      f"{t_total:.1f} s single-threaded"}
+                      f"{t_total:.1f} s single-threaded",
+            **host_info()}
+
+
+def host_info():
+    """CPU model, memory and hardware threads of the host the CPU baseline ran on."""
+    model, mem_kib = None, None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal"):
+                    mem_kib = int(line.split()[1])
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"cpu_model": model, "ram_GiB": round(mem_kib / 2 ** 20, 1) if mem_kib else None,
+            "host_threads": os.cpu_count(), "threads_usable": usable}
 
 
 def bench_batch(count, plain_budget_s=8.0, reps=3, cpu_budget_s=10.0):
@@ -374,6 +403,69 @@ def bench_tiled(sv_ref, tape, n, reps):
     return out
 
 
+def distinct_xmasks(ham, n):
+    """Number of distinct X/Y masks among the Hamiltonian's Pauli words (the
+    Z-only words share the mask 0): one read of the state each is the
+    algorithmic traffic of the expectation."""
+    masks = set()
+    for t in ham.terms:
+        fs = t.factors if t.kind == "tensor" else [t]
+        m = 0
+        for f in fs:
+            if f.name in ("PauliX", "PauliY"):
+                m |= 1 << (n - 1 - f.wires[0])
+        masks.add(m)
+    return len(masks)
+
+
+def bench_measurements(sv, ham, uniforms, n, reps=3):
+    """<H> of the 100-term Hamiltonian and inverse-CDF sampling of the
+    uniforms on the state in sv: one warm call each, then `reps` calls timed
+    with HIP events on the engine's stream (the best is reported) and by the
+    wall clock (host checks, table staging and the copy of the indices
+    back included).  Algorithmic bytes: one read of the state per distinct
+    x-mask for <H>; one read of the state plus 16 bytes per sample for the
+    sampler."""
+    S = (1 << n) * 16
+    out = {}
+    e = sv.expval(ham)
+    dev, wall = [], []
+    for _ in range(reps):
+        sv.synchronize()
+        t0 = time.perf_counter()
+        sv.event_record(0)
+        e = sv.expval(ham)
+        sv.event_record(1)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(sv.event_elapsed(0, 1))
+    nm = distinct_xmasks(ham, n)
+    ms = min(dev)
+    out["expval_100_terms"] = {"value": e, "ms": round(ms, 2), "wall_ms": round(min(wall), 2),
+                               "distinct_xmasks": nm, "algorithmic_GB": round(nm * S / 1e9, 2),
+                               "GBs": round(nm * S / (ms / 1e3) / 1e9, 1),
+                               "hbm_frac": round(nm * S / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    idx = sv.sample(uniforms)
+    dev, wall = [], []
+    for _ in range(reps):
+        sv.synchronize()
+        t0 = time.perf_counter()
+        sv.event_record(0)
+        idx = sv.sample(uniforms)
+        sv.event_record(1)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(sv.event_elapsed(0, 1))
+    m = len(uniforms)
+    ms = min(dev)
+    b = S + 16 * m
+    out["samples"] = {"count": m, "ms": round(ms, 2), "wall_ms": round(min(wall), 2),
+                      "algorithmic_GB": round(b / 1e9, 2), "GBs": round(b / (ms / 1e3) / 1e9, 1),
+                      "hbm_frac": round(b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "first_indices": [int(i) for i in idx[:4]]}
+    log(f"<H> = {e:.12f}: {out['expval_100_terms']['ms']} ms ({out['expval_100_terms']['GBs']} GB/s); "
+        f"{m} samples: {out['samples']['ms']} ms ({out['samples']['GBs']} GB/s)")
+    return out
+
+
 def bench_precision(n_big=33, n_ref=30):
     """Precision form: the same seeded 50-layer gate mix, consecutive gates
     fused into 5-qubit 32x32 blocks applied to a complex64 state on FP32
@@ -496,6 +588,45 @@ def bench_surrogate(samples=1 << 20, epochs=3):
                                            "plain_calls": c0, "screened_calls": c1}}
         log(f"surrogate {tag}: R2 E {out[tag]['heldout_R2_E']}, screen {gain}, "
             f"GD plain {h0[-1]:.4f} vs screened {h1[-1]:.4f}")
+    out["hea16x6_trajectory"] = bench_surrogate_trajectory()
+    return out
+
+
+def bench_surrogate_trajectory(budget=120):
+    """The learned-surrogate form at its stated size (16 qubits, 6 layers,
+    192 angles, 192 -> 512 -> 512 -> 193 on BF16 matrix cores) with the
+    network trained around the optimiser's trajectory: every 8 accepted
+    steps 4096 points theta + 0.05 N(0, 1) are evaluated by the batched
+    adjoint engine (complex128) and the network is retrained on them; each
+    step it screens 256 step lengths along -grad E and the exact simulation
+    verifies its pick.  Compared with plain gradient descent (same start,
+    eta = 0.05, halved on rejection) at equal budgets of exact calls inside
+    the optimisation loop; the training evaluations are reported apart."""
+    from lightning_unification_amd import surrogate as S
+    exact = S.Problem(16, 6, batch=4096)
+    th0 = S.sample_thetas(1, exact.nparams, 100)[0]
+    t0 = time.perf_counter()
+    tr_plain = []
+    S.optimize(exact, th0, target=-1e9, max_calls=budget, eta=0.05, trace=tr_plain)
+    t_plain = time.perf_counter() - t0
+    sur = S.Surrogate(exact.nparams, kind="local", seed=3)
+    t0 = time.perf_counter()
+    tr_loc = []
+    _, loop_calls, data_calls, _ = S.optimize_local(exact, th0, target=-1e9, max_calls=budget, surrogate=sur,
+                                                    trace=tr_loc)
+    t_loc = time.perf_counter() - t0
+    sur.close()
+    exact.close()
+    e_plain = tr_plain[-1][1]
+    out = {"budget_loop_calls": budget, "start_E": tr_plain[0][1],
+           "plain": {"E": e_plain, "calls": tr_plain[-1][0], "accepted": len(tr_plain), "s": round(t_plain, 2)},
+           "screened": {"E": tr_loc[-1][1], "loop_calls": loop_calls, "training_evaluations": data_calls,
+                        "accepted": len(tr_loc), "s": round(t_loc, 2),
+                        "loop_calls_to_reach_plain_final_E": S.calls_to_reach(tr_loc, e_plain)},
+           "E_at_calls": {str(c): {"plain": S.energy_at(tr_plain, c), "screened": S.energy_at(tr_loc, c)}
+                          for c in (10, 20, 40, 80, budget)}}
+    log(f"surrogate trajectory: plain {e_plain:.4f} vs screened {tr_loc[-1][1]:.4f} at {budget} loop calls "
+        f"({data_calls} training evaluations)")
     return out
 
 
@@ -562,8 +693,8 @@ def main():
     ap.add_argument("--no-tiled", action="store_true", help="skip the tile-pass measurement")
     ap.add_argument("--no-next", action="store_true",
                     help="skip the VJP, sparse-H and Metropolis measurements")
-    ap.add_argument("--mode", choices=["gates", "copy", "stream", "batch", "precision", "surrogate", "c1",
-                                       "next"],
+    ap.add_argument("--mode", choices=["gates", "copy", "stream", "measure", "batch", "precision", "surrogate",
+                                       "c1", "next"],
                     default="gates",
                     help="copy: device-to-device copies of a 4 GiB state, the plain copy "
                          "kernel used to scale rocprof FETCH_SIZE / WRITE_SIZE")
@@ -581,6 +712,17 @@ def main():
             b.copy_from(a)
         b.synchronize()
         print(json.dumps({"mode": "copy", "bytes_per_copy": (1 << 28) * 16, "copies": 3}))
+        return
+    if args.mode == "measure":
+        # the benchmark circuit's final state (tile passes), then the
+        # measurements alone: the run to profile k_pauli_tile and the sampler
+        n = args.qubits
+        sv = capi.DeviceStateVector(n, np.complex128)
+        sv.apply_tape_tiled(random_layers(n, 50, seed=1002))
+        res = bench_measurements(sv, random_hamiltonian(n, 100, seed=1002),
+                                 sample_uniforms(args.shots, seed=1002), n)
+        sv.close()
+        print(json.dumps({"mode": "measure", **res}))
         return
     if args.mode == "stream":
         print(json.dumps({"mode": "stream", "stream": bench_stream(args.qubits)}))
@@ -658,13 +800,7 @@ def main():
     per_gate_gbs = {g: round(b / (ms / 1e3) / 1e9, 1) for g, (b, ms) in per_gate_kind.items()}
 
     # measurements on the final state
-    t0 = time.perf_counter()
-    e = sv.expval(ham)
-    t_expval = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    idx = sv.sample(uniforms)
-    t_sample = time.perf_counter() - t0
-    log(f"<H> = {e:.12f} ({t_expval*1e3:.1f} ms), {len(idx)} samples ({t_sample*1e3:.1f} ms)")
+    meas = bench_measurements(sv, ham, uniforms, n)
     tiled = None if args.no_tiled else bench_tiled(sv, tape, n, args.warmup + args.steps)
     sv.close()
 
@@ -699,8 +835,8 @@ def main():
                      "traffic": traffic, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "per_gate_GBs": per_gate_gbs,
-        "expval_100_terms": {"value": e, "ms": round(t_expval * 1e3, 2)},
-        "samples": {"count": len(idx), "ms": round(t_sample * 1e3, 2)},
+        "expval_100_terms": meas["expval_100_terms"],
+        "samples": meas["samples"],
         "tiled_circuit": tiled,
         "c1_adjoint": c1,
         "next_rows": next_rows,

@@ -201,8 +201,11 @@ int pl_adjoint_jacobian(pl_sv_t sv, int nops, const int *ids,
 enum { PL_FUSE_RENORM = 1, PL_FUSE_NO_MFMA = 2 };
 
 /* The operations of StateVectorBase::applyOperations (StateVectorBase.hpp:84-120),
- * with consecutive operations whose wires together span at most max_wires
- * (1..5) wires multiplied into one dense block first (greedy, in tape order).
+ * multiplied into dense blocks of at most max_wires (1..5) wires first.  A
+ * block grows from the first operation not yet placed by scanning ahead in
+ * tape order: an operation joins when its wires fit and it shares no wire
+ * with an operation the scan passed over, so operations only move ahead of
+ * operations on disjoint wires, with which they commute.
  * On complex64 states with n >= 12 every block is widened to 5 wires and
  * applied on FP32 matrix cores; elsewhere blocks use the dense-matrix kernels.
  * flags: PL_FUSE_RENORM renormalises to unit norm (FP64 norm; on the matrix-
@@ -214,17 +217,20 @@ int pl_sv_apply_ops_fused(pl_sv_t sv, int nops, const int *ids, const int *nwire
                           const int *nparams, const double *params, int max_wires,
                           int flags, int *nblocks);
 /* The fusion plan of pl_sv_apply_ops_fused, computed on the host only (no
- * device is touched): for block b, first_op[b], block_nops[b], nbits[b]
- * (0 = one operation applied unfused), bits[8 b ..] ascending bit positions,
- * and mats[2 * 1024 b ..] the 2^nbits x 2^nbits row-major matrix (interleaved
+ * device is touched): order[] lists the tape indices of all operations
+ * (state preparation excluded) in application order, block after block; for
+ * block b, first_op[b] is the offset of its first operation in order[],
+ * block_nops[b] its operation count, nbits[b] its width (0 = one operation
+ * applied unfused), bits[8 b ..] ascending bit positions, and
+ * mats[2 * 1024 b ..] the 2^nbits x 2^nbits row-major matrix (interleaved
  * double complex) in the block-local index whose bit m is bit position
  * bits[8 b + m].  Blocks narrower than pad_to are widened with identity bits.
  * Fails with PL_ERR_INVALID, and sets *nblocks, when capacity is too small. */
 int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
                  const int *wires, const uint8_t *inverse, const int *nparams,
                  const double *params, int max_wires, int pad_to, int capacity,
-                 int *nblocks, int *first_op, int *block_nops, int *nbits,
-                 int *bits, double *mats);
+                 int *nblocks, int *first_op, int *block_nops, int *order,
+                 int *nbits, int *bits, double *mats);
 /* StateVectorBase::applyOperations (StateVectorBase.hpp:84-120) executed as
  * tile passes: the tape is split into passes, each of which loads tiles of
  * 2^tile_bits amplitudes (the low_bits lowest bit positions plus
@@ -236,12 +242,14 @@ int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
  * count; low_bits in [0, 6].  tile_bits = 0 and low_bits = -1 pick the
  * defaults of the precision (2^10 amplitudes and 3 low bits for complex128,
  * 2^11 and 4 for complex64, clamped to the state); with tile_bits = 0 a state
- * of fewer than 9 qubits runs gate by gate.  *npasses (may be NULL) receives
- * the number of launches. */
+ * of fewer than 9 qubits runs gate by gate.  flags: PL_FUSE_RENORM rescales
+ * the state to unit norm at the end, the norm accumulated in FP64 (the
+ * precision form's renormalisation for complex64 states).  *npasses (may be
+ * NULL) receives the number of launches. */
 int pl_sv_apply_ops_tiled(pl_sv_t sv, int nops, const int *ids, const int *nwires,
                           const int *wires, const uint8_t *inverse,
                           const int *nparams, const double *params, int tile_bits,
-                          int low_bits, int *npasses);
+                          int low_bits, int flags, int *npasses);
 /* The pass plan of pl_sv_apply_ops_tiled, computed on the host only: for
  * pass p, pass_nops[p] operations, pass_tiled[p] (0 = one operation applied
  * by the per-gate kernels), pass_bits[16 p ..] the higher tile bit positions

@@ -207,16 +207,16 @@ class DeviceStateVector:
             int(max_wires), flags, ctypes.byref(nb)))
         return nb.value
 
-    def apply_tape_tiled(self, tape, tile_bits=0, low_bits=-1):
+    def apply_tape_tiled(self, tape, tile_bits=0, low_bits=-1, renorm=False):
         """Apply a tape as tile passes (pl_sv_apply_ops_tiled); returns the
         number of launches.  tile_bits=0 / low_bits=-1: the defaults of the
-        precision."""
+        precision; renorm rescales to unit norm at the end (FP64 norm)."""
         ids, nw, wires, inv, npar, par = tape.encode()
         npass = ctypes.c_int()
         _check(load().pl_sv_apply_ops_tiled(
             self.h, len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
             _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)), _p(_f64(par)),
-            int(tile_bits), int(low_bits), ctypes.byref(npass)))
+            int(tile_bits), int(low_bits), PL_FUSE_RENORM if renorm else 0, ctypes.byref(npass)))
         return npass.value
 
     def norm2(self):
@@ -365,26 +365,29 @@ class DeviceStateVector:
 
 def fuse_tape(num_qubits, tape, max_wires=5, pad_to=0):
     """Host-only fusion plan (pl_fuse_tape): list of blocks
-    (first_op, nops, bits or None for an unfused op, matrix or None)."""
+    (tape op indices in application order, bits or None for an unfused op,
+    matrix or None)."""
     ids, nw, wires, inv, npar, par = tape.encode()
     cap = max(1, len(ids))
     nb = ctypes.c_int()
     first, nops, nbits = (np.zeros(cap, np.int32) for _ in range(3))
+    order = np.zeros(cap, np.int32)
     bits = np.zeros(8 * cap, np.int32)
     mats = np.zeros(2048 * cap, np.float64)
     _check(load().pl_fuse_tape(int(num_qubits), len(ids), _p(_i32(ids)), _p(_i32(nw)), _p(_i32(wires)),
                                _p(inv if inv.size else np.zeros(1, np.uint8)), _p(_i32(npar)),
                                _p(_f64(par)), int(max_wires), int(pad_to), cap, ctypes.byref(nb),
-                               _p(first), _p(nops), _p(nbits), _p(bits), _p(mats)))
+                               _p(first), _p(nops), _p(order), _p(nbits), _p(bits), _p(mats)))
     out = []
     for b in range(nb.value):
         k = int(nbits[b])
+        ops = order[first[b]:first[b] + nops[b]].tolist()
         if k == 0:
-            out.append((int(first[b]), int(nops[b]), None, None))
+            out.append((ops, None, None))
             continue
         d = 1 << k
         m = mats[2048 * b: 2048 * b + 2 * d * d].view(np.complex128).reshape(d, d).copy()
-        out.append((int(first[b]), int(nops[b]), bits[8 * b: 8 * b + k].tolist(), m))
+        out.append((ops, bits[8 * b: 8 * b + k].tolist(), m))
     return out
 
 

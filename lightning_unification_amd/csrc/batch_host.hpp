@@ -11,6 +11,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels_batch.hpp"
@@ -88,6 +89,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
     std::vector<uint32_t> term_off, term_cnt;
     std::vector<uint64_t> par_off;
     uint64_t npar = 0;
+    std::vector<double> norms; // <psi|psi> of each circuit's final state, last replay
 
     int W = 0;
     hipStream_t stream = nullptr;
@@ -162,7 +164,7 @@ template <class T> struct BatchPlan final : BatchPlanBase {
         HIP_OK(hipMalloc(&dparams, std::max<uint64_t>(npar, 1) * sizeof(double)));
         HIP_OK(hipMalloc(&dgrads, std::max<uint64_t>(npar, 1) * sizeof(double)));
         HIP_OK(hipMalloc(&dcs, std::max<uint64_t>(npar, 1) * sizeof(double2)));
-        HIP_OK(hipMalloc(&denergy, std::max(nc, 1) * sizeof(double)));
+        HIP_OK(hipMalloc(&denergy, 2 * std::max(nc, 1) * sizeof(double))); // energies, then <psi|psi>
         if (!terms.empty()) {
             HIP_OK(hipMemcpy(dterms, terms.data(), terms.size() * sizeof(PauliTerm), hipMemcpyHostToDevice));
             HIP_OK(hipMemcpy(dxm, xmasks.data(), xmasks.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -332,8 +334,8 @@ template <class T> struct BatchPlan final : BatchPlanBase {
         const size_t tile_bytes = (size_t{1} << W) * sizeof(V);
         hipLaunchKernelGGL(k_zero, dim3((npar + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
                            dgrads, npar);
-        hipLaunchKernelGGL(k_zero, dim3((nc + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
-                           denergy, static_cast<uint64_t>(nc));
+        hipLaunchKernelGGL(k_zero, dim3((2 * nc + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
+                           denergy, static_cast<uint64_t>(2 * nc));
         hipLaunchKernelGGL(k_batch_trig, dim3((npar + kBlock - 1) / kBlock + 1), dim3(kBlock), 0, stream,
                            dparams, npar, dcs);
         for (const auto &ls : seq) {
@@ -354,7 +356,8 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                 break;
             case LaunchSpec::Obs:
                 hipLaunchKernelGGL((k_batch_obs<T>), grid, dim3(kBlock), 0, stream, arena, dcircs,
-                                   ls.L, entangler ? 1 : 0, dterms, dxm, denergy);
+                                   ls.L, entangler ? 1 : 0, dterms, dxm, denergy,
+                                   denergy + nc);
                 break;
             case LaunchSpec::RingAdj:
                 // undo the entangler of layer l: buffers (l+1)%2 -> l%2 for lambda and H
@@ -430,11 +433,30 @@ template <class T> struct BatchPlan final : BatchPlanBase {
     void run(const double *params, double *energies, double *grads) override {
         HIP_OK(hipMemcpyAsync(dparams, params, npar * sizeof(double), hipMemcpyHostToDevice, stream));
         HIP_OK(hipGraphLaunch(exec, stream));
+        norms.resize(nc);
         HIP_OK(hipMemcpyAsync(energies, denergy, nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipMemcpyAsync(norms.data(), denergy + nc, nc * sizeof(double), hipMemcpyDeviceToHost, stream));
         if (grads) {
             HIP_OK(hipMemcpyAsync(grads, dgrads, npar * sizeof(double), hipMemcpyDeviceToHost, stream));
         }
         HIP_OK(hipStreamSynchronize(stream));
+        if constexpr (!std::is_same_v<T, double>) {
+            // complex64: the FP64 norm of each final state removes the norm
+            // drift of FP32 rounding from E = <psi|H|psi> / <psi|psi> and from
+            // its gradient (d<psi|psi>/dtheta = 0 for a unitary circuit)
+            uint64_t k = 0;
+            for (int j = 0; j < nc; j++) {
+                const double inv = norms[j] > 0 ? 1.0 / norms[j] : 1.0;
+                energies[j] *= inv;
+                const uint64_t m = static_cast<uint64_t>(depth[j]) * nq[j] * npat;
+                if (grads) {
+                    for (uint64_t a = 0; a < m; a++) {
+                        grads[k + a] *= inv;
+                    }
+                }
+                k += m;
+            }
+        }
     }
     int tile_width() const override { return W; }
     int launches() const override { return static_cast<int>(seq.size()) + 3; }

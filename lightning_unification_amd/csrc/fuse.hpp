@@ -1,6 +1,11 @@
-// Gate fusion for the precision path: consecutive operations of a tape whose
-// wires together span at most `max_wires` wires are multiplied into one dense
-// block (2^k x 2^k, k <= 5).  Each block's matrix is built on the host by
+// Gate fusion for the precision path: operations of a tape whose wires
+// together span at most `max_wires` wires are multiplied into one dense block
+// (2^k x 2^k, k <= 5).  A block is grown from the first operation not yet
+// placed by scanning ahead in tape order: an operation joins when its bits
+// fit the block and it shares no bit with any operation the scan has passed
+// over, so it only moves ahead of operations on disjoint bits, with which it
+// commutes (the rule of the tile-pass scheduler).  Each block's matrix is
+// built on the host by
 // applying the very group ops the GPU kernels apply (lower_gate) to the 2^k
 // basis vectors of a k-qubit local system, so fused and unfused runs share
 // one definition of every gate.  An operation on more than `max_wires` wires
@@ -21,9 +26,8 @@
 namespace plamd {
 
 struct FusedBlock {
-    int first_op = 0;
-    int nops = 0;
-    bool fused = true;              // false: apply operation first_op unfused
+    std::vector<int> ops;           // tape indices, in application order
+    bool fused = true;              // false: apply operation ops[0] unfused
     std::vector<int> bits;          // ascending bit positions
     std::vector<std::complex<double>> U; // 2^k x 2^k row-major in the block-local index
 };
@@ -70,33 +74,42 @@ struct TapeView {
     const double *params;
 };
 
+constexpr int kFuseLookahead = 512; // operations scanned ahead per block
+
 inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires, int pad_to) {
     if (max_wires < 1 || max_wires > 5) {
         throw InvalidArg("fusion width must be in [1, 5]");
     }
     pad_to = std::min(pad_to, n);
     std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
+    std::vector<uint64_t> mask(t.nops, 0);
     for (int i = 0; i < t.nops; i++) {
         if (t.nwires[i] < 0 || t.nparams[i] < 0) {
             throw InvalidArg("negative wire or parameter count");
         }
         woff[i + 1] = woff[i] + t.nwires[i];
         poff[i + 1] = poff[i] + t.nparams[i];
+        if (t.ids[i] == -1) {
+            continue;
+        }
+        for (int a = 0; a < t.nwires[i]; a++) {
+            const int w = t.wires[woff[i] + a];
+            if (w < 0 || w >= n) {
+                throw InvalidArg("wire index out of range");
+            }
+            mask[i] |= uint64_t{1} << (n - 1 - w);
+        }
     }
     std::vector<FusedBlock> out;
-    std::vector<int> cur; // bit positions of the open block
-    int start = 0;
 
-    auto close = [&](int end) {
-        if (end <= start) {
-            return;
+    auto build = [&](FusedBlock &b, uint64_t bitmask) {
+        for (int p = 0; p < n; p++) {
+            if ((bitmask >> p) & 1U) {
+                b.bits.push_back(p);
+            }
         }
-        FusedBlock b;
-        b.first_op = start;
-        b.nops = end - start;
-        b.bits = cur;
         for (int p = 0; static_cast<int>(b.bits.size()) < pad_to && p < n; p++) {
-            if (std::find(b.bits.begin(), b.bits.end(), p) == b.bits.end()) {
+            if (!((bitmask >> p) & 1U)) {
                 b.bits.push_back(p);
             }
         }
@@ -104,10 +117,7 @@ inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires
         const int k = static_cast<int>(b.bits.size());
         const size_t dim = size_t{1} << k;
         std::vector<GroupOp<double>> ops;
-        for (int i = start; i < end; i++) {
-            if (t.ids[i] == -1) {
-                continue; // state preparation: skipped, as by applyOperations
-            }
+        for (int i : b.ops) {
             std::vector<int> lw(t.nwires[i]);
             for (int a = 0; a < t.nwires[i]; a++) {
                 const int bit = n - 1 - t.wires[woff[i] + a];
@@ -129,48 +139,43 @@ inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires
                 b.U[r * dim + c] = col[r];
             }
         }
-        out.push_back(std::move(b));
-        cur.clear();
     };
 
-    for (int i = 0; i < t.nops; i++) {
-        if (t.ids[i] == -1) {
-            continue;
+    std::vector<char> placed(t.nops, 0);
+    int pos = 0;
+    while (true) {
+        while (pos < t.nops && (placed[pos] || t.ids[pos] == -1)) { // state preparation: skipped
+            pos++;
         }
-        std::vector<int> bits;
-        for (int a = 0; a < t.nwires[i]; a++) {
-            const int w = t.wires[woff[i] + a];
-            if (w < 0 || w >= n) {
-                throw InvalidArg("wire index out of range");
-            }
-            bits.push_back(n - 1 - w);
+        if (pos >= t.nops) {
+            break;
         }
-        if (static_cast<int>(bits.size()) > max_wires) {
-            close(i);
-            FusedBlock b;
-            b.first_op = i;
-            b.nops = 1;
+        placed[pos] = 1;
+        FusedBlock b;
+        b.ops.push_back(pos);
+        if (t.nwires[pos] > max_wires) { // applied unfused
             b.fused = false;
             out.push_back(std::move(b));
-            start = i + 1;
             continue;
         }
-        std::vector<int> uni = cur;
-        for (int b : bits) {
-            if (std::find(uni.begin(), uni.end(), b) == uni.end()) {
-                uni.push_back(b);
+        uint64_t cur = mask[pos], blocked = 0;
+        const uint64_t all = n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1;
+        for (int j = pos + 1; j < t.nops && j <= pos + kFuseLookahead && blocked != all; j++) {
+            if (placed[j] || t.ids[j] == -1) {
+                continue;
+            }
+            const uint64_t u = cur | mask[j];
+            if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && __builtin_popcountll(u) <= max_wires) {
+                cur = u;
+                placed[j] = 1;
+                b.ops.push_back(j);
+            } else {
+                blocked |= mask[j];
             }
         }
-        if (static_cast<int>(uni.size()) > max_wires) {
-            close(i);
-            start = i;
-            uni = bits;
-            std::sort(uni.begin(), uni.end());
-            uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
-        }
-        cur = uni;
+        build(b, cur);
+        out.push_back(std::move(b));
     }
-    close(t.nops);
     return out;
 }
 

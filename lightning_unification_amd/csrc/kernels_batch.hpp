@@ -294,16 +294,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(v2<T> *__restrict__ arena
     }
 }
 
-// H psi into buffer 2 + f and E += Re<psi|H psi> (FP64 atomics per block),
-// where psi sits in buffer f = depth mod 2 when layers end with an
-// entangler (each one flips the buffer) and f = 0 otherwise.
+// H psi into buffer 2 + f, E += Re<psi|H psi> and N += <psi|psi> (FP64
+// atomics per block), where psi sits in buffer f = depth mod 2 when layers
+// end with an entangler (each one flips the buffer) and f = 0 otherwise.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
                                                       const BCirc *__restrict__ circs,
                                                       const BLaunch L, int ent,
                                                       const PauliTerm *__restrict__ terms,
                                                       const uint64_t *__restrict__ xmasks,
-                                                      double *__restrict__ energy) {
+                                                      double *__restrict__ energy,
+                                                      double *__restrict__ norm2) {
     uint64_t q;
     const int ci = find_circ(L, blockIdx.x, q);
     const BCirc &c = circs[ci];
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
     const int pbuf = ent ? (c.depth & 1) : 0;
     const int hbuf = 2 + pbuf;
     const uint64_t y = q * kBlock + threadIdx.x;
-    double e = 0;
+    double e = 0, pn = 0;
     if (y < len) {
         const v2<T> *psi = arena + c.off + pbuf * len;
         double hr = 0, hi = 0;
@@ -345,10 +346,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
         arena[c.off + hbuf * len + y] = hv;
         const v2<T> p = psi[y];
         e = double(p.x) * hr + double(p.y) * hi;
+        pn = double(p.x) * double(p.x) + double(p.y) * double(p.y);
     }
-    const double2 s = block_sum2(double2{e, 0});
+    const double2 s = block_sum2(double2{e, pn});
     if (threadIdx.x == 0) {
         atomicAdd(energy + ci, s.x);
+        atomicAdd(norm2 + ci, s.y);
     }
 }
 

@@ -164,7 +164,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_lane(v2<T> *__restrict__ sv, ui
 // loads, applies every group that lies in the tile, and writes the whole tile
 // back, so HBM sees whole lines in both directions.
 constexpr int kTileLow = 8;   // tile spans amplitude bits 0..7
-constexpr int kTileMaxHi = 4; // LDS: 256 * 16 * 16 B = 64 KiB for complex128
+// A lowered op with at most four members pins at most four bits (the
+// double excitation), and this path needs one of them below bit 3, so at
+// most three pins are high.  LDS: 256 * 8 * 16 B = 32 KiB for complex128.
+constexpr int kTileMaxHi = 3;
 
 template <class T, int M> struct TileOp {
     Geom hi;                 // pins >= kTileLow (ascending), no fixed bits

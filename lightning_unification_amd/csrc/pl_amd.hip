@@ -136,8 +136,7 @@ template <class T> struct Engine {
         case 0: hipLaunchKernelGGL((k_group_tile<T, M, 0, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
         case 1: hipLaunchKernelGGL((k_group_tile<T, M, 1, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
         case 2: hipLaunchKernelGGL((k_group_tile<T, M, 2, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-        case 3: hipLaunchKernelGGL((k_group_tile<T, M, 3, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
-        default: hipLaunchKernelGGL((k_group_tile<T, M, 4, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
+        default: hipLaunchKernelGGL((k_group_tile<T, M, 3, DIAG>), dim3(grid), dim3(kBlock), 0, h->stream, sv, ntiles, to); break;
         }
     }
 
@@ -346,7 +345,7 @@ template <class T> struct Engine {
         for (size_t i = 0; i < blocks.size(); i++) {
             const auto &b = blocks[i];
             if (!b.fused) {
-                const int o = b.first_op;
+                const int o = b.ops[0];
                 gate(sv, t.ids[o], t.wires + woff[o], t.nwires[o], t.inverse[o] != 0, t.params + poff[o],
                      t.nparams[o]);
                 continue;
@@ -403,6 +402,19 @@ template <class T> struct Engine {
             *keep = std::move(blocks);
         }
         return nb;
+    }
+
+    // psi <- psi / ||psi|| with the norm accumulated in FP64 (one read pass and
+    // one scaling pass)
+    void renormalize(V *sv) {
+        const double nrm = dot(sv, sv).real();
+        const double sc = nrm > 0 ? 1.0 / std::sqrt(nrm) : 1.0;
+        auto *d = static_cast<double *>(buf(h, kBufScalar, sizeof(double)));
+        HIP_OK(hipMemcpyAsync(d, &sc, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        hipLaunchKernelGGL((k_scale<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream, sv, len(), d);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(h->stream));
     }
 
     // ---- tile passes: many gates per read and write of the state --------------
@@ -1669,13 +1681,19 @@ int pl_sv_apply_ops_fused(pl_sv_t h, int nops, const int *ids, const int *nwires
 
 int pl_sv_apply_ops_tiled(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
                           const uint8_t *inverse, const int *nparams, const double *params,
-                          int tile_bits, int low_bits, int *npasses) {
+                          int tile_bits, int low_bits, int flags, int *npasses) {
     return dispatch(h, [&](auto &E) {
         if (nops < 0 || (nops > 0 && (!ids || !nwires || !wires || !inverse || !nparams))) {
             throw InvalidArg("null tape argument");
         }
+        if (flags & ~PL_FUSE_RENORM) {
+            throw InvalidArg("unknown flag for the tiled executor");
+        }
         const TapeView t{nops, ids, nwires, wires, inverse, nparams, params};
         const int np = E.apply_tiled(E.psi(), t, tile_bits, low_bits);
+        if (flags & PL_FUSE_RENORM) {
+            E.renormalize(E.psi());
+        }
         if (npasses) {
             *npasses = np;
         }
@@ -1717,8 +1735,8 @@ int pl_tile_plan(int num_qubits, int nops, const int *ids, const int *nwires, co
 
 int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires, const int *wires,
                  const uint8_t *inverse, const int *nparams, const double *params, int max_wires,
-                 int pad_to, int capacity, int *nblocks, int *first_op, int *block_nops, int *nbits,
-                 int *bits, double *mats) {
+                 int pad_to, int capacity, int *nblocks, int *first_op, int *block_nops, int *order,
+                 int *nbits, int *bits, double *mats) {
     return guarded([&] {
         if (num_qubits < 1 || num_qubits > 62 || nops < 0 || !nblocks) {
             throw InvalidArg("invalid argument");
@@ -1729,10 +1747,15 @@ int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires, co
         if (*nblocks > capacity) {
             throw InvalidArg("capacity too small for " + std::to_string(*nblocks) + " blocks");
         }
+        int at = 0;
         for (size_t b = 0; b < blocks.size(); b++) {
             const auto &B = blocks[b];
-            if (first_op) first_op[b] = B.first_op;
-            if (block_nops) block_nops[b] = B.nops;
+            if (first_op) first_op[b] = at;
+            if (block_nops) block_nops[b] = static_cast<int>(B.ops.size());
+            for (int i : B.ops) {
+                if (order) order[at] = i;
+                at++;
+            }
             if (nbits) nbits[b] = B.fused ? static_cast<int>(B.bits.size()) : 0;
             for (size_t m = 0; bits && B.fused && m < B.bits.size(); m++) {
                 bits[8 * b + m] = B.bits[m];

@@ -57,10 +57,12 @@ class Problem:
             p.close()
 
 
-def features(theta, kind="angle"):
+def features(theta, kind="angle", center=None, scale=1.0):
     theta = np.asarray(theta, np.float64)
     if kind == "periodic":
         return np.concatenate([np.cos(theta), np.sin(theta)], axis=-1).astype(np.float32)
+    if kind == "local":  # offsets from a trust-region centre, in units of its radius
+        return ((theta - center) / scale).astype(np.float32)
     return (theta / math.pi - 1.0).astype(np.float32)
 
 
@@ -85,6 +87,8 @@ class Surrogate:
         self.net = capi.Mlp((d0, hidden[0], hidden[1], nparams + 1), max_batch, seed)
         self.mu = np.zeros(nparams + 1)
         self.sd = np.ones(nparams + 1)
+        self.center = np.zeros(nparams)
+        self.scale = 1.0
 
     def targets(self, E, G):
         return ((np.column_stack([E, G]) - self.mu) / self.sd).astype(np.float32)
@@ -98,12 +102,12 @@ class Surrogate:
         # the energy is one output of nparams + 1: weight it like all gradients together
         w[0] = float(self.nparams) if energy_weight is None else energy_weight
         self.net.set_loss_weights(w / w.sum())
-        self.net.load_data(features(theta, self.kind), self.targets(E, G))
+        self.net.load_data(features(theta, self.kind, self.center, self.scale), self.targets(E, G))
         steps = max(1, theta.shape[0] // batch)
         return [self.net.train(steps, batch, lr) for _ in range(epochs)]
 
     def predict(self, theta):
-        y = self.net.predict(features(np.atleast_2d(theta), self.kind)).astype(np.float64)
+        y = self.net.predict(features(np.atleast_2d(theta), self.kind, self.center, self.scale)).astype(np.float64)
         y = y * self.sd + self.mu
         return y[:, 0], y[:, 1:]
 
@@ -116,18 +120,21 @@ def candidate_steps(eta, count=256, span=4.0):
     return eta * np.exp2(np.linspace(-span, span, count))
 
 
-def optimize(problem, theta0, target, max_calls=400, eta=0.05, surrogate=None, candidates=256):
+def optimize(problem, theta0, target, max_calls=400, eta=0.05, surrogate=None, candidates=256, trace=None):
     """Gradient descent from theta0 until E <= target or max_calls exact calls.
 
     Without a surrogate: theta - eta * g, verified exactly; a rejected step
     halves eta.  With one: the surrogate predicts E for `candidates` step
     sizes along -g and the exact simulation verifies the best; a rejected
     pick falls back to the plain step.  Every accepted step is exact.
-    Returns (E history of accepted steps, exact calls used, final theta)."""
+    Returns (E history of accepted steps, exact calls used, final theta);
+    `trace`, if a list, receives (exact calls so far, E) per accepted step."""
     start_calls = problem.calls
     theta = np.array(theta0, np.float64)
     (E,), (g,) = problem.exact(theta[None])
     hist = [E]
+    if trace is not None:
+        trace.append((1, E))
     while E > target and problem.calls - start_calls < max_calls:
         tried = False
         if surrogate is not None:
@@ -141,6 +148,8 @@ def optimize(problem, theta0, target, max_calls=400, eta=0.05, surrogate=None, c
                 theta, E, g = cand[pick], E1, g1
                 eta = float(steps[pick])  # the plain fallback continues from the screened step
                 hist.append(E)
+                if trace is not None:
+                    trace.append((problem.calls - start_calls, E))
                 continue
         while problem.calls - start_calls < max_calls:
             cand = theta - eta * g
@@ -148,11 +157,94 @@ def optimize(problem, theta0, target, max_calls=400, eta=0.05, surrogate=None, c
             if E1 < E:
                 theta, E, g = cand, E1, g1
                 hist.append(E)
+                if trace is not None:
+                    trace.append((problem.calls - start_calls, E))
                 break
             eta *= 0.5
             if tried:
                 break
     return np.array(hist), problem.calls - start_calls, theta
+
+
+def optimize_local(problem, theta0, target, max_calls, surrogate, sigma=0.05, samples=4096, refresh=8,
+                   epochs=6, batch=512, lr=1e-3, candidates=256, eta=0.05, seed=77, trace=None):
+    """Gradient descent screened by a surrogate trained around the iterate.
+
+    Every `refresh` accepted steps the batched engine evaluates `samples`
+    points theta + sigma * N(0, 1) around the current iterate (the training
+    evaluations, counted apart) and the network, in features centred there,
+    is retrained on them.  Each step it predicts E for `candidates` points
+    theta - s * g / |g| with s from sigma / 16 to 4 sigma, and the exact
+    simulation verifies the best; a rejected pick falls back to the plain
+    step (theta - eta g, eta halved on rejection).  Every accepted step is
+    exact.  Returns (E history of accepted steps, loop exact calls, training
+    evaluations, final theta); `trace`, if a list, receives (loop exact calls
+    so far, E) per accepted step."""
+    start_calls = problem.calls
+    theta = np.array(theta0, np.float64)
+    (E,), (g,) = problem.exact(theta[None])
+    loop_calls = 1
+    data_calls = 0
+    hist = [E]
+    if trace is not None:
+        trace.append((1, E))
+    rs = np.random.default_rng(seed)
+    since = refresh
+    while E > target and loop_calls < max_calls:
+        if since >= refresh:
+            th = theta[None, :] + sigma * rs.standard_normal((samples, theta.size))
+            Ed, Gd = problem.exact(th)
+            data_calls += samples
+            surrogate.center = theta.copy()
+            surrogate.scale = sigma
+            surrogate.fit(th, Ed, Gd, epochs=epochs, batch=batch, lr=lr)
+            since = 0
+        gn = float(np.linalg.norm(g))
+        steps = sigma * np.exp2(np.linspace(-4.0, 2.0, candidates))
+        cand = theta[None, :] - (steps / max(gn, 1e-300))[:, None] * g[None, :]
+        e_hat, _ = surrogate.predict(cand)
+        pick = int(np.argmin(e_hat))
+        (E1,), (g1,) = problem.exact(cand[pick][None])
+        loop_calls += 1
+        if E1 < E:
+            theta, E, g = cand[pick], E1, g1
+            hist.append(E)
+            if trace is not None:
+                trace.append((loop_calls, E))
+            since += 1
+            continue
+        while loop_calls < max_calls:
+            cand1 = theta - eta * g
+            (E1,), (g1,) = problem.exact(cand1[None])
+            loop_calls += 1
+            if E1 < E:
+                theta, E, g = cand1, E1, g1
+                hist.append(E)
+                if trace is not None:
+                    trace.append((loop_calls, E))
+                since += 1
+                break
+            eta *= 0.5
+        since = refresh  # the model missed: retrain around the new iterate
+    assert problem.calls - start_calls == loop_calls + data_calls
+    return np.array(hist), loop_calls, data_calls, theta
+
+
+def energy_at(trace, calls):
+    """Energy of the last accepted step within `calls` exact calls."""
+    e = None
+    for c, E in trace:
+        if c <= calls:
+            e = E
+    return e
+
+
+def calls_to_reach(trace, target):
+    """Exact calls until the first accepted step at or below `target`."""
+    for c, E in trace:
+        if E <= target:
+            return c
+    return None
 
 
 def r2(pred, true):

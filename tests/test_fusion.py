@@ -1,7 +1,9 @@
 """Gate fusion plan (pl_fuse_tape, host only): applying the fused blocks in
 order must reproduce the oracle's unfused run of the same tape, for every
 fusion width, with and without padding to 5-bit blocks; blocks respect the
-width, cover the tape in order, and are unitary."""
+width and are unitary; every operation (state preparation aside) is applied
+exactly once, and an operation applied before an earlier one of the tape
+shares no wire with it (they commute)."""
 import numpy as np
 import pytest
 
@@ -50,21 +52,32 @@ def test_fused_blocks_reproduce_tape(oracle, max_wires, pad_to):
     exp = oracle.apply_tape(psi.copy(), without_prep(tape))
     blocks = capi.fuse_tape(n, tape, max_wires, pad_to)
     got = psi.copy()
-    covered = 0
-    for first, nops, bits, U in blocks:
-        assert first == covered
-        covered += nops
+    applied = []
+    for ops, bits, U in blocks:
+        applied += ops
         if bits is None:
+            assert len(ops) == 1
             t1 = Tape()
-            t1.ops = [op for op in tape.ops[first:first + nops] if op[0] in GATE_ID]
+            t1.ops = [tape.ops[ops[0]]]
             got = oracle.apply_tape(got, t1)
             continue
         assert len(bits) == max(pad_to, len(bits)) and len(bits) <= max(max_wires, pad_to)
         assert bits == sorted(bits)
         assert np.allclose(U @ U.conj().T, np.eye(U.shape[0]), atol=1e-13)
         got = apply_block(got, n, bits, U)
-    assert covered == len(tape.ops)
+    gates = [i for i, op in enumerate(tape.ops) if op[0] in GATE_ID]
+    assert sorted(applied) == gates
+    check_overtaking(tape, applied)
     assert np.max(np.abs(got - exp)) < 1e-12
+
+
+def check_overtaking(tape, applied):
+    """Operations applied out of tape order act on disjoint wires."""
+    pos = {i: k for k, i in enumerate(applied)}
+    for a in applied:
+        for b in applied:
+            if a < b and pos[a] > pos[b]:
+                assert not set(tape.ops[a][1]) & set(tape.ops[b][1]), (a, b)
 
 
 def without_prep(tape):
@@ -80,8 +93,12 @@ def test_fusion_counts_and_errors():
     from lightning_unification_amd import capi
     tape = random_layers(30, 50, seed=1002)
     blocks = capi.fuse_tape(30, tape, 5)
-    assert sum(b[1] for b in blocks) == len(tape)
-    assert len(blocks) < len(tape) / 2
+    applied = [i for b in blocks for i in b[0]]
+    assert sorted(applied) == list(range(len(tape)))
+    check_overtaking(tape, applied)
+    # moving operations past commuting ones packs about six per 5-wire block
+    # (consecutive operations alone give about three)
+    assert len(blocks) < len(tape) / 5
     with pytest.raises(ValueError):
         capi.fuse_tape(30, tape, 6)
     with pytest.raises(ValueError):

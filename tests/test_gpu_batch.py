@@ -58,15 +58,20 @@ def test_batch_matches_oracle(oracle, dtype, pattern, ent, width):
     if width:
         assert plan.tile_width == width
     e, g = plan.run(params)
-    tol = 1e-12 if dtype == np.complex128 else 2e-5
-    assert rel_err(e, exp_e) < tol
-    assert rel_err(g, exp_g) < tol
+    # complex64: the energy bar of the batching form (1e-6); gradients, which
+    # have no stated bar, 1e-5
+    tol = 1e-12 if dtype == np.complex128 else 1e-6
+    gtol = 1e-12 if dtype == np.complex128 else 1e-5
+    ee, ge = rel_err(e, exp_e), rel_err(g, exp_g)
+    print(f"energy error {ee:.3e}, gradient error {ge:.3e}", flush=True)
+    assert ee < tol
+    assert ge < gtol
     # a second replay with new angles reuses the graph
     params2 = Rng(78, width).uniform(npar) * 2 * np.pi
     exp_e2, exp_g2 = _expected(oracle, SHAPES, pattern, ent, params2, obs)
     e2, g2 = plan.run(params2)
     assert rel_err(e2, exp_e2) < tol
-    assert rel_err(g2, exp_g2) < tol
+    assert rel_err(g2, exp_g2) < gtol
     e3 = plan.run(params2, grads=False)
     assert rel_err(e3, exp_e2) < tol
     plan.close()

@@ -21,15 +21,22 @@ def _capi():
 
 
 def wire_tuples(n, k, limit=None):
-    ts = list(itertools.permutations(range(n), k))
-    if limit and len(ts) > limit:
-        rs = np.random.default_rng(k * 100 + n)
-        ts = [ts[i] for i in sorted(rs.choice(len(ts), limit, replace=False))]
-    return ts
+    """Ordered k-tuples of distinct wires: all of them, or `limit` distinct
+    ones drawn at random (without listing all n!/(n-k)! of them first)."""
+    count = 1
+    for i in range(k):
+        count *= n - i
+    if not limit or count <= limit:
+        return list(itertools.permutations(range(n), k))
+    rs = np.random.default_rng(k * 100 + n)
+    seen = set()
+    while len(seen) < limit:
+        seen.add(tuple(int(w) for w in rs.permutation(n)[:k]))
+    return sorted(seen)
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("n", [1, 3, 7, 11])
+@pytest.mark.parametrize("n", [1, 3, 7, 11, 13])
 def test_all_gates_all_wires(oracle, dtype, n):
     capi = _capi()
     rs = np.random.default_rng(n)
@@ -51,6 +58,27 @@ def test_all_gates_all_wires(oracle, dtype, n):
                     worst = max(worst, e)
                     assert e < TOL[dtype], (g, wires, inv, e)
     print("worst rel err", worst)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_four_wire_gates_on_the_tile_path(oracle, dtype):
+    """Two-member group ops with one member bit inside a 128-byte line and
+    three at bit 8 or above: the LDS tile kernel with four high pins (13
+    qubits: wire w is bit 12 - w, so wires 10..12 are bits 0..2 and wires
+    0..4 are bits 8..12)."""
+    capi = _capi()
+    n = 13
+    rs = np.random.default_rng(413)
+    tuples = [(0, 1, 2, 12), (12, 0, 3, 1), (4, 11, 2, 0), (1, 3, 10, 2), (2, 4, 0, 11)]
+    for g in ("DoubleExcitation", "DoubleExcitationMinus", "DoubleExcitationPlus"):
+        for wires in tuples:
+            for inv in (False, True):
+                p = list(rs.uniform(-7, 7, 1))
+                s0 = random_state(n, int(rs.integers(1 << 30)), dtype)
+                exp = oracle.apply_tape(s0.copy(), Tape().add(g, wires, p, inv))
+                with capi.DeviceStateVector.from_array(s0) as sv:
+                    sv.apply(g, wires, inv, p)
+                    assert rel_err(sv.state(), exp) < TOL[dtype], (g, wires, inv)
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
