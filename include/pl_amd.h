@@ -128,6 +128,17 @@ int pl_expval(pl_sv_t sv, const int *prog, const double *coeffs,
               const double *mats, double *out);
 int pl_var(pl_sv_t sv, const int *prog, const double *coeffs,
            const double *mats, double *out);
+/* pl_expval without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Takes Pauli words and Hamiltonians of Pauli words (named PauliX / PauliY /
+ * PauliZ / Identity factors) and fails with PL_ERR_INVALID otherwise.  Its
+ * term tables and partial sums live in a fixed 2 MiB device array per GPU
+ * (held until the process ends); an observable needing more fails with
+ * PL_ERR_INVALID (a 100-term Hamiltonian needs about 0.2 MiB).  Calls that
+ * overlap on one GPU share that array and race. */
+int pl_expval_noalloc(pl_sv_t sv, const int *prog, const double *coeffs,
+                      const double *mats, double *out);
 /* Measurements::expval/var(name, wires) (MeasurementsLQubit.hpp:167-181,298-317). */
 int pl_expval_named(pl_sv_t sv, int gate_id, const int *wires, int nwires,
                     double *out);

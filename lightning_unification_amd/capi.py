@@ -32,7 +32,7 @@ EXPORTS = [
     "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
     "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train", "pl_vjp", "pl_expval_sparse",
     "pl_var_sparse", "pl_generate_mcmc_samples", "pl_sv_apply_ops_tiled", "pl_tile_plan",
-    "pl_transfer_counters",
+    "pl_transfer_counters", "pl_expval_noalloc",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -292,6 +292,13 @@ class DeviceStateVector:
         prog, coeffs, mats = self._obs(ob)
         out = ctypes.c_double()
         _check(load().pl_expval(self.h, _p(prog), _p(coeffs), _p(mats), ctypes.byref(out)))
+        return out.value
+
+    def expval_noalloc(self, ob):
+        """pl_expval_noalloc: Pauli-word observables, no device allocation."""
+        prog, coeffs, mats = self._obs(ob)
+        out = ctypes.c_double()
+        _check(load().pl_expval_noalloc(self.h, _p(prog), _p(coeffs), _p(mats), ctypes.byref(out)))
         return out.value
 
     def var(self, ob):

@@ -1841,6 +1841,55 @@ int pl_expval(pl_sv_t h, const int *prog, const double *coeffs, const double *ma
         *out = E.expval(rd.one(), false);
     });
 }
+// ---- allocation-free expectation value --------------------------------------
+// Device scratch of pl_expval_noalloc: the term tables and partial sums of
+// one call.  Each GPU the library runs on holds its own copy until the
+// process ends; every call on that GPU reuses it, so calls that overlap on
+// one GPU race.  Nothing in it needs clearing: each call writes every
+// partial it reads.
+constexpr size_t kNoAllocScratchBytes = size_t{2} << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+unsigned char *noalloc_scratch() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        throw DeviceError("device index out of range for the allocation-free scratch");
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+} // namespace
+
+int pl_expval_noalloc(pl_sv_t h, const int *prog, const double *coeffs, const double *mats, double *out) {
+    return dispatch(h, [&](auto &E) {
+        if (!prog || !out) {
+            throw InvalidArg("null argument");
+        }
+        ObsReader rd{prog + 1, coeffs, mats};
+        const ObsNode o = rd.one();
+        std::vector<PauliWord> ws;
+        std::vector<double> cs;
+        if (!E.as_pauli_sum(o, ws, cs)) {
+            throw InvalidArg("the allocation-free expectation value takes Pauli words and Hamiltonians of "
+                             "Pauli words only (other observables need a state-sized scratch)");
+        }
+        *out = E.pauli_expval(E.psi(), ws, cs, noalloc_scratch(), kNoAllocScratchBytes);
+    });
+}
+
 int pl_var(pl_sv_t h, const int *prog, const double *coeffs, const double *mats, double *out) {
     return dispatch(h, [&](auto &E) {
         ObsReader rd{prog + 1, coeffs, mats};
