@@ -1,19 +1,26 @@
 """Benchmark: gate-application HBM throughput on one MI355X.
 
 Workload (one "step"): the seeded 50-layer random circuit on 30 qubits in
-complex128 (16 GiB state, ~950 gates drawn from {Hadamard, T, RX, RY, RZ, Rot,
-CRX, CRZ, IsingXX, IsingZZ, SWAP, Toffoli}), applied to a state resident in
-HBM.  value = algorithmic bytes moved by the gates / device time, where a gate
-that can change a fraction f of the amplitudes moves 2 * f * 2^n * 16 bytes.
-Every launch is bracketed by HIP events on the engine's stream (the stream
-the kernels run on).  Also reported: the 100-term Hamiltonian expectation and
-10^6 inverse-CDF samples on the final state, the same circuit through the
-tile-pass executor (many gates per read and write of the state), the VJP,
-sparse-Hamiltonian and Metropolis entry points (next_rows), and a CPU
+complex128 (16 GiB state, 990 gates drawn from {Hadamard, T, RX, RY, RZ, Rot,
+CRX, CRZ, IsingXX, IsingZZ, SWAP, Toffoli}), applied gate by gate to a state
+resident in HBM.  value = algorithmic bytes moved by the gates / device time,
+where a gate that can change a fraction f of the amplitudes moves
+2 * f * 2^n * 16 bytes.  Every launch is bracketed by HIP events on the
+engine's stream (the stream the kernels run on).  Also reported, each in its
+own object of the JSON line: the 100-term Hamiltonian expectation and 10^6
+inverse-CDF samples on the final state (event-timed, GB/s of their
+algorithmic bytes), the same circuit through the tile-pass executor, the
+20-qubit adjoint Jacobian, the VJP, sparse-Hamiltonian and Metropolis entry
+points (next_rows), the batching form (4096 circuits, forward + adjoint), the
+precision form (33 qubits, complex64), the learned surrogate, and a CPU
 baseline (the reference's Lightning-Qubit kernels compiled from source, or the
 oracle restatement) on a bounded sample of the same gates.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N]
+Modes for single measurements: copy (the copy kernel that calibrates the
+traffic counters), stream (the device's streaming ceilings), measure (<H>
+and sampling on the circuit's state), batch, precision, surrogate, c1, next.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits N] [--mode M]
 """
 import argparse
 import json
@@ -467,9 +474,11 @@ def bench_measurements(sv, ham, uniforms, n, reps=3):
 
 
 def bench_precision(n_big=33, n_ref=30):
-    """Precision form: the same seeded 50-layer gate mix, consecutive gates
-    fused into 5-qubit 32x32 blocks applied to a complex64 state on FP32
-    matrix cores, renormalised in FP64 after every block.
+    """Precision form: the same seeded 50-layer gate mix fused into 5-qubit
+    32x32 blocks (operations move ahead only past operations on disjoint
+    wires) applied to a complex64 state on FP32 matrix cores, renormalised in
+    FP64 after every block; and the same circuit as complex64 tile passes,
+    renormalised in FP64 at the end.
     Error reference at n_ref qubits: the fused complex64 run against the
     unfused complex128 run of the same circuit (|d<H>|, ||d psi||_2, norm
     drift without renormalisation).  Throughput at n_big qubits (64 GiB)."""
@@ -491,15 +500,18 @@ def bench_precision(n_big=33, n_ref=30):
     s64.close()
     s64 = capi.DeviceStateVector(n_ref, np.complex64)
     s64.apply_tape_tiled(tape)
+    drift_t = abs(s64.norm2() - 1.0)
+    s64.reset()
+    s64.apply_tape_tiled(tape, renorm=True)
     et = s64.expval(ham)
     dist_t = s64.distance(s128)
-    drift_t = abs(s64.norm2() - 1.0)
     s64.close()
     s128.close()
     out["error_vs_c128"] = {"qubits": n_ref, "blocks": nb, "abs_dH": abs(e64 - e128),
                             "H_c128": e128, "state_l2": dist,
                             "norm_drift_without_renorm": drift,
-                            "tiled_c64": {"abs_dH": abs(et - e128), "state_l2": dist_t, "norm_drift": drift_t}}
+                            "tiled_c64": {"abs_dH": abs(et - e128), "state_l2": dist_t,
+                                          "norm_drift_without_renorm": drift_t}}
     log(f"precision n={n_ref}: |dH| {abs(e64 - e128):.2e}, |dpsi| {dist:.2e}, drift {drift:.2e}")
 
     # throughput at n_big
@@ -516,7 +528,10 @@ def bench_precision(n_big=33, n_ref=30):
     S = (1 << n_big) * 8
     gate_bytes = sum(2 * touched_fraction(op[0]) * S for op in tape.ops)
     block_bytes = nb * 2 * S  # every block reads and writes the state once
+    # a 32x32 complex block per group of 32 amplitudes: 8 * 32 * 32 FLOP of
+    # complex arithmetic, executed as three real 32x32 products (Gauss)
     flops = nb * (1 << (n_big - 5)) * 8 * 32 * 32
+    mfma_flops = nb * (1 << (n_big - 5)) * 3 * 2 * 32 * 32
     t0 = time.perf_counter()
     e = sv.expval(ham)
     t_h = time.perf_counter() - t0
@@ -524,7 +539,7 @@ def bench_precision(n_big=33, n_ref=30):
     sv.reset()
     sv.synchronize()
     sv.timer_start()
-    npass = sv.apply_tape_tiled(tape)
+    npass = sv.apply_tape_tiled(tape, renorm=True)
     ms_t = sv.timer_stop()
     norm_t = sv.norm2()
     sv.close()
@@ -536,10 +551,13 @@ def bench_precision(n_big=33, n_ref=30):
     out["c64_fused"] = {"qubits": n_big, "gates": len(tape), "blocks": nb, "ms": round(ms, 1),
                         "block_GBs": round(block_bytes / (ms / 1e3) / 1e9, 1),
                         "gate_equivalent_GBs": round(gate_bytes / (ms / 1e3) / 1e9, 1),
-                        "mfma_TFLOPs": round(flops / (ms / 1e3) / 1e12, 2),
+                        "complex_TFLOPs": round(flops / (ms / 1e3) / 1e12, 2),
+                        "mfma_TFLOPs": round(mfma_flops / (ms / 1e3) / 1e12, 2),
+                        "mfma_frac": round(mfma_flops / (ms / 1e3) / 1e12 / 157.3, 4),
+                        "hbm_frac": round(block_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                         "H": e, "H_ms": round(t_h * 1e3, 1), "norm": norm}
     log(f"precision n={n_big}: {nb} blocks in {ms:.0f} ms, "
-        f"{block_bytes / (ms / 1e3) / 1e9:.0f} GB/s, {flops / (ms / 1e3) / 1e12:.1f} TFLOP/s")
+        f"{block_bytes / (ms / 1e3) / 1e9:.0f} GB/s, {mfma_flops / (ms / 1e3) / 1e12:.1f} TFLOP/s")
     return out
 
 

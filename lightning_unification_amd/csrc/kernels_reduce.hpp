@@ -314,9 +314,12 @@ struct PauliTileGeom {
     int high[kPauliTileHigh];     // their positions, ascending (all >= 3)
     int nlow;                     // low bits in S: min(3, n)
 };
+// The terms of a group are sorted into buckets by the part of zyl that the
+// thread's own elements differ in (pauli_bucket); bucket k is the term range
+// [tb[k], tb[k + 1]) (4 buckets for a pair group, 8 for the diagonal one).
 struct PauliTileGroup {
     uint32_t xl; // x-mask in the compressed tile index (0: diagonal terms)
-    int t0, t1;  // its terms
+    int tb[9];
 };
 // A term of a pair group contributes s_e (cr Re z + ci Im z) for the pair
 // (e, f = e ^ x), z = conj(psi_e) psi_f and s_e = (-1)^popcount(e & zy): with
@@ -331,39 +334,65 @@ struct PauliTileTerm {
     int pad;
 };
 
+constexpr int kPauliPerThread = 8; // amplitudes a thread loads per tile (2^11 / 256)
+constexpr int kPauliPairs = kPauliPerThread / 2;
+
+// Tile-local element e with a zero inserted at bit lb (the pair's low end).
+__host__ __device__ inline uint32_t pauli_insert0(uint32_t e, int lb) {
+    const uint32_t m = (1U << lb) - 1;
+    return ((e & ~m) << 1) | (e & m);
+}
+// Thread t of a workgroup owns the elements t + 256 j (diagonal group, j < 8)
+// or the pairs whose low ends are pauli_insert0(t + 256 j, lb) (j < 4).  They
+// differ from the first one in bits fixed by j alone, so the sign of a term at
+// element j is its sign at the first element times (-1)^popcount(j & k), k
+// made of the term's zyl bits at those positions: the bucket of the term.
+__host__ __device__ inline int pauli_bucket(uint32_t xl, uint32_t zyl) {
+    if (xl == 0) {
+        return static_cast<int>((zyl >> 8) & 7U);
+    }
+    int lb = 0;
+    while (!((xl >> lb) & 1U)) {
+        lb++;
+    }
+    const uint32_t d1 = pauli_insert0(256, lb), d2 = pauli_insert0(512, lb);
+    return ((zyl & d1) ? 1 : 0) | ((zyl & d2) ? 2 : 0);
+}
+
+// v with its sign flipped where bit 0 of par is set
+__device__ __forceinline__ double pauli_flip(double v, uint32_t par) {
+    return __longlong_as_double(__double_as_longlong(v) ^ (static_cast<unsigned long long>(par) << 63));
+}
+
 // One pass of the Pauli-sum expectation over the whole state.  Persistent
 // workgroups walk the tiles q = blockIdx.x, blockIdx.x + gridDim.x, ...; each
 // thread holds its share of the next tile in registers while the current one
 // is evaluated from LDS, so the state streams in behind the arithmetic.
-// Per tile: the terms' parities over the bits outside S are constant and are
-// folded into the sign of (cr, ci) once; per pair and term one AND, one
-// popcount and two FMAs remain.
-constexpr int kPauliPerThread = 8;              // amplitudes a thread loads per tile (2^11 / 256)
-constexpr int kPauliMaxTerms = 128;             // terms of one pass held in LDS
-struct PauliPassTerm {
-    double cr, ci;
-    uint32_t zyl;
-    int pad;
-};
-
+//
+// Per group a thread forms its values (|psi_e|^2 of 8 elements, or z of 4
+// pairs) and their Walsh-Hadamard transform W[k] = sum_j (-1)^popcount(j & k)
+// v_j.  A term of bucket k then contributes (-1)^p (its coefficient times
+// W[k]) with p its parity at the thread's first element (the tile's bits
+// outside S included): one AND, one popcount, a sign flip and two FMAs per
+// term and thread, not per pair.  Terms and groups are read by uniform index
+// (scalar loads).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const PauliTileGeom geo, int ngroups,
                                                        const PauliTileGroup *__restrict__ groups,
-                                                       const PauliTileTerm *__restrict__ terms, int t_first,
-                                                       int nterms, double2 *__restrict__ partial) {
+                                                       const PauliTileTerm *__restrict__ terms,
+                                                       double2 *__restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
-    __shared__ PauliPassTerm tt[kPauliMaxTerms];
     const int nS = geo.nlow + geo.nhigh;
     const uint32_t tsz = 1U << nS;
     const uint32_t lowmask = (1U << geo.nlow) - 1;
-    const int per = static_cast<int>((tsz + kBlock - 1) / kBlock); // <= kPauliPerThread
+    const uint32_t tid = threadIdx.x;
     // the tile-local part of each loaded element's state index
     uint64_t off[kPauliPerThread];
 #pragma unroll
     for (int k = 0; k < kPauliPerThread; k++) {
-        const uint32_t e = threadIdx.x + k * kBlock;
+        const uint32_t e = tid + k * kBlock;
         uint64_t g = e & lowmask;
         for (int b = 0; b < geo.nhigh; b++) {
             g |= static_cast<uint64_t>((e >> (geo.nlow + b)) & 1U) << geo.high[b];
@@ -385,24 +414,20 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
     if (q < ntiles) {
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (k < per && threadIdx.x + k * kBlock < tsz) {
+            if (tid + k * kBlock < tsz) {
                 nxt[k] = ld_nt<T>(&sv[base | off[k]]);
             }
         }
     }
     double acc = 0;
     for (; q < ntiles; q += gridDim.x) {
-        __syncthreads(); // the previous tile and its term table are no longer read
+        const uint64_t cur = base; // this tile's bits outside S
+        __syncthreads();           // the previous tile is no longer read
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (k < per && threadIdx.x + k * kBlock < tsz) {
-                tile[threadIdx.x + k * kBlock] = nxt[k];
+            if (tid + k * kBlock < tsz) {
+                tile[tid + k * kBlock] = nxt[k];
             }
-        }
-        for (int t = threadIdx.x; t < nterms; t += kBlock) {
-            const PauliTileTerm tm = terms[t_first + t];
-            const bool neg = (__popcll(base & tm.zyo) & 1U) != 0;
-            tt[t] = PauliPassTerm{neg ? -tm.cr : tm.cr, neg ? -tm.ci : tm.ci, tm.zyl, 0};
         }
         __syncthreads();
         const uint64_t qn = q + gridDim.x;
@@ -410,42 +435,83 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             base = tile_base(qn);
 #pragma unroll
             for (int k = 0; k < kPauliPerThread; k++) {
-                if (k < per && threadIdx.x + k * kBlock < tsz) {
+                if (tid + k * kBlock < tsz) {
                     nxt[k] = ld_nt<T>(&sv[base | off[k]]);
                 }
             }
         }
         for (int gi = 0; gi < ngroups; gi++) {
-            const PauliTileGroup gr = groups[gi];
-            const int t0 = gr.t0 - t_first, t1 = gr.t1 - t_first;
-            if (gr.xl == 0) {
-                for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
-                    const v2<T> a = tile[e];
-                    const double p = double(a.x) * double(a.x) + double(a.y) * double(a.y);
-                    double s = 0;
-                    for (int t = t0; t < t1; t++) {
-                        const double c = (__popc(e & tt[t].zyl) & 1U) ? -tt[t].cr : tt[t].cr;
-                        s += c;
+            const uint32_t xl = groups[gi].xl;
+            if (xl == 0) {
+                double w[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const uint32_t e = tid + j * kBlock;
+                    w[j] = 0;
+                    if (e < tsz) {
+                        const v2<T> a = tile[e];
+                        w[j] = double(a.x) * double(a.x) + double(a.y) * double(a.y);
                     }
-                    acc = fma(p, s, acc);
+                }
+#pragma unroll
+                for (int h = 1; h < 8; h <<= 1) {
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        if (!(j & h)) {
+                            const double u = w[j], v = w[j | h];
+                            w[j] = u + v;
+                            w[j | h] = u - v;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int t1 = groups[gi].tb[k + 1];
+                    for (int t = groups[gi].tb[k]; t < t1; t++) {
+                        const PauliTileTerm tm = terms[t];
+                        const uint32_t par = __popc(tid & tm.zyl) + __popcll(cur & tm.zyo);
+                        acc += pauli_flip(tm.cr * w[k], par);
+                    }
                 }
                 continue;
             }
-            const int lb = __ffs(gr.xl) - 1;
-            const uint32_t lowm = (1U << lb) - 1;
-            for (uint32_t pe = threadIdx.x; pe < (tsz >> 1); pe += kBlock) {
-                const uint32_t e = ((pe & ~lowm) << 1) | (pe & lowm);
-                const v2<T> a = tile[e], c = tile[e ^ gr.xl];
-                // z = conj(a) * c
-                const double zr = double(a.x) * double(c.x) + double(a.y) * double(c.y);
-                const double zi = double(a.x) * double(c.y) - double(a.y) * double(c.x);
-                double sa = 0, sb = 0;
-                for (int t = t0; t < t1; t++) {
-                    const bool neg = (__popc(e & tt[t].zyl) & 1U) != 0;
-                    sa += neg ? -tt[t].cr : tt[t].cr;
-                    sb += neg ? -tt[t].ci : tt[t].ci;
+            const int lb = __ffs(xl) - 1;
+            const uint32_t npairs = tsz >> 1;
+            const uint32_t e0 = pauli_insert0(tid, lb);
+            double zr[kPauliPairs], zi[kPauliPairs];
+#pragma unroll
+            for (int j = 0; j < kPauliPairs; j++) {
+                const uint32_t pe = tid + j * kBlock;
+                zr[j] = zi[j] = 0;
+                if (pe < npairs) {
+                    const uint32_t e = pauli_insert0(pe, lb);
+                    const v2<T> a = tile[e], c = tile[e ^ xl];
+                    // z = conj(a) * c
+                    zr[j] = double(a.x) * double(c.x) + double(a.y) * double(c.y);
+                    zi[j] = double(a.x) * double(c.y) - double(a.y) * double(c.x);
                 }
-                acc = fma(sa, zr, fma(sb, zi, acc));
+            }
+#pragma unroll
+            for (int h = 1; h < kPauliPairs; h <<= 1) {
+#pragma unroll
+                for (int j = 0; j < kPauliPairs; j++) {
+                    if (!(j & h)) {
+                        const double ur = zr[j], vr = zr[j | h], ui = zi[j], vi = zi[j | h];
+                        zr[j] = ur + vr;
+                        zr[j | h] = ur - vr;
+                        zi[j] = ui + vi;
+                        zi[j | h] = ui - vi;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPauliPairs; k++) {
+                const int t1 = groups[gi].tb[k + 1];
+                for (int t = groups[gi].tb[k]; t < t1; t++) {
+                    const PauliTileTerm tm = terms[t];
+                    const uint32_t par = __popc(e0 & tm.zyl) + __popcll(cur & tm.zyo);
+                    acc += pauli_flip(fma(tm.ci, zi[k], tm.cr * zr[k]), par);
+                }
             }
         }
     }

@@ -571,10 +571,10 @@ template <class T> struct Engine {
         }
         const int nlow = std::min(3, n());
         const uint64_t lowm = (uint64_t{1} << nlow) - 1;
-        // work items: an x-mask with at most kPauliMaxTerms of its terms
+        // work items: an x-mask with all of its terms
         struct Item {
             uint64_t x;
-            std::vector<PauliTerm> terms;
+            const std::vector<PauliTerm> *terms;
         };
         std::vector<Item> items;
         std::vector<uint64_t> wide;
@@ -583,10 +583,7 @@ template <class T> struct Engine {
                 wide.push_back(kv.first);
                 continue;
             }
-            for (size_t a = 0; a < kv.second.size(); a += kPauliMaxTerms) {
-                const size_t b = std::min(kv.second.size(), a + kPauliMaxTerms);
-                items.push_back(Item{kv.first, {kv.second.begin() + a, kv.second.begin() + b}});
-            }
+            items.push_back(Item{kv.first, &kv.second});
         }
         // widest high parts first; the diagonal items (x = 0) fit any pass
         std::stable_sort(items.begin(), items.end(), [&](const Item &a, const Item &b) {
@@ -594,7 +591,6 @@ template <class T> struct Engine {
         });
         struct Pass {
             uint64_t high = 0;
-            size_t nterms = 0;
             std::vector<const Item *> its;
         };
         std::vector<Pass> passes;
@@ -607,10 +603,8 @@ template <class T> struct Engine {
             std::vector<const Item *> left;
             for (const Item *it : rest) {
                 const uint64_t xh = it->x & ~lowm;
-                if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh &&
-                    ps.nterms + it->terms.size() <= static_cast<size_t>(kPauliMaxTerms)) {
+                if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh) {
                     ps.high |= xh;
-                    ps.nterms += it->terms.size();
                     ps.its.push_back(it);
                 } else {
                     left.push_back(it);
@@ -619,9 +613,17 @@ template <class T> struct Engine {
             passes.push_back(std::move(ps));
             rest.swap(left);
         }
+        // every tile as large as the LDS budget allows: the high set is padded
+        // with the lowest free positions (a pass whose masks need one high bit
+        // would otherwise walk 2^(n-4) tiles of 16 amplitudes)
+        for (auto &ps : passes) {
+            for (int b = nlow; b < n() && __builtin_popcountll(ps.high) < kPauliTileHigh; b++) {
+                ps.high |= uint64_t{1} << b;
+            }
+        }
         // per pass: geometry, groups and compressed terms, all staged together
         std::vector<PauliTileGeom> geos;
-        std::vector<std::pair<size_t, size_t>> gspan, tspan; // offsets into the tables
+        std::vector<std::pair<size_t, size_t>> gspan; // offsets into the group table
         std::vector<PauliTileGroup> gtab;
         std::vector<PauliTileTerm> ttab;
         for (size_t pi = 0; pi < passes.size(); pi++) {
@@ -647,11 +649,14 @@ template <class T> struct Engine {
                 return r;
             };
             geos.push_back(g);
-            const size_t g0 = gtab.size(), t0 = ttab.size();
+            const size_t g0 = gtab.size();
             for (const Item *it : ps.its) {
                 const uint64_t x = it->x;
-                PauliTileGroup gr{compress(x), static_cast<int>(ttab.size()), 0};
-                for (const auto &pt : it->terms) {
+                PauliTileGroup gr{};
+                gr.xl = compress(x);
+                const int nb = x ? kPauliPairs : 2 * kPauliPairs;
+                std::vector<std::vector<PauliTileTerm>> bucket(nb);
+                for (const auto &pt : *it->terms) {
                     const uint32_t zyl = compress(pt.zy & smask);
                     double cr = pt.coeff, ci = 0;
                     if (x != 0) { // see PauliTileTerm
@@ -663,13 +668,16 @@ template <class T> struct Engine {
                             continue; // contributes exactly zero
                         }
                     }
-                    ttab.push_back(PauliTileTerm{pt.zy & ~smask, cr, ci, zyl, 0});
+                    bucket[pauli_bucket(gr.xl, zyl)].push_back(PauliTileTerm{pt.zy & ~smask, cr, ci, zyl, 0});
                 }
-                gr.t1 = static_cast<int>(ttab.size());
+                for (int k = 0; k < nb; k++) {
+                    gr.tb[k] = static_cast<int>(ttab.size());
+                    ttab.insert(ttab.end(), bucket[k].begin(), bucket[k].end());
+                }
+                gr.tb[nb] = static_cast<int>(ttab.size());
                 gtab.push_back(gr);
             }
             gspan.emplace_back(g0, gtab.size() - g0);
-            tspan.emplace_back(t0, ttab.size() - t0);
         }
         std::vector<PauliTerm> wtab;
         std::vector<size_t> wstart;
@@ -711,8 +719,7 @@ template <class T> struct Engine {
             const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(ntiles, kReduceBlocks));
             hipLaunchKernelGGL((k_pauli_tile<T>), dim3(grid), dim3(kBlock), (size_t{1} << nS) * sizeof(V),
                                h->stream, sv, ntiles, g, static_cast<int>(gspan[pi].second),
-                               dgroups + gspan[pi].first, dtterms, static_cast<int>(tspan[pi].first),
-                               static_cast<int>(tspan[pi].second), part + total);
+                               dgroups + gspan[pi].first, dtterms, part + total);
             HIP_OK(hipGetLastError());
             total += grid;
         }

@@ -43,8 +43,9 @@ def test_noalloc_rejects_what_needs_scratch():
         h = np.array([[1.0, 0.5], [0.5, -1.0]])
         with pytest.raises(ValueError):
             sv.expval_noalloc(Obs.hermitian(h, [2]))
-        # more Pauli terms than the fixed scratch holds
-        many = random_hamiltonian(8, 60000, seed=6)
+        # more Pauli terms than the fixed scratch holds (32 bytes per term
+        # against 2 MiB)
+        many = random_hamiltonian(8, 100000, seed=6)
         with pytest.raises(ValueError):
             sv.expval_noalloc(many)
         # the array is still usable afterwards
