@@ -245,12 +245,14 @@ int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
 /* StateVectorBase::applyOperations (StateVectorBase.hpp:84-120) executed as
  * tile passes: the tape is split into passes, each of which loads tiles of
  * 2^tile_bits amplitudes (the low_bits lowest bit positions plus
- * tile_bits - low_bits higher ones) into LDS, applies every operation of the
- * pass there and writes the tiles back, so the state is read and written once
- * per pass.  Operations move ahead of earlier ones only across operations on
- * disjoint wires, with which they commute.  Operations a pass cannot hold are
- * applied by the per-gate kernels.  tile_bits in [9, 12] and <= the qubit
- * count; low_bits in [0, 6].  tile_bits = 0 and low_bits = -1 pick the
+ * tile_bits - low_bits higher ones) into the registers of one wavefront,
+ * applies every operation of the pass there and writes the tiles back, so the
+ * state is read and written once per pass.  Operations move ahead of earlier
+ * ones only across operations on disjoint wires, with which they commute.
+ * Operations a pass cannot hold are applied by the per-gate kernels.
+ * tile_bits in [9, 12] ([9, 11] for complex128: a 2^12-amplitude tile does
+ * not fit a wavefront's registers) and <= the qubit count; low_bits in
+ * [0, 6].  tile_bits = 0 and low_bits = -1 pick the
  * defaults of the precision (2^10 amplitudes and 3 low bits for complex128,
  * 2^11 and 4 for complex64, clamped to the state); with tile_bits = 0 a state
  * of fewer than 9 qubits runs gate by gate.  flags: PL_FUSE_RENORM rescales

@@ -318,7 +318,8 @@ struct PauliTileGeom {
 // thread's own elements differ in (pauli_bucket); bucket k is the term range
 // [tb[k], tb[k + 1]) (4 buckets for a pair group, 8 for the diagonal one).
 struct PauliTileGroup {
-    uint32_t xl; // x-mask in the compressed tile index (0: diagonal terms)
+    uint32_t xl;   // x-mask in the compressed tile index (0: diagonal terms)
+    uint32_t used; // bit k: bucket k holds terms
     int tb[9];
 };
 // A term of a pair group contributes s_e (cr Re z + ci Im z) for the pair
@@ -376,7 +377,8 @@ __device__ __forceinline__ double pauli_flip(double v, uint32_t par) {
 // outside S included): one AND, one popcount, a sign flip and two FMAs per
 // term and thread, not per pair.  Terms and groups are read by uniform index
 // (scalar loads).
-template <class T>
+// FULL: every tile holds 2^11 amplitudes (no element or pair is out of range)
+template <class T, bool FULL>
 __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const PauliTileGeom geo, int ngroups,
                                                        const PauliTileGroup *__restrict__ groups,
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
     const int nS = geo.nlow + geo.nhigh;
-    const uint32_t tsz = 1U << nS;
+    const uint32_t tsz = FULL ? (1U << (3 + kPauliTileHigh)) : (1U << nS);
     const uint32_t lowmask = (1U << geo.nlow) - 1;
     const uint32_t tid = threadIdx.x;
     // the tile-local part of each loaded element's state index
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
     if (q < ntiles) {
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (tid + k * kBlock < tsz) {
+            if (FULL || tid + k * kBlock < tsz) {
                 nxt[k] = ld_nt<T>(&sv[base | off[k]]);
             }
         }
@@ -425,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
         __syncthreads();           // the previous tile is no longer read
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (tid + k * kBlock < tsz) {
+            if (FULL || tid + k * kBlock < tsz) {
                 tile[tid + k * kBlock] = nxt[k];
             }
         }
@@ -435,23 +437,44 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             base = tile_base(qn);
 #pragma unroll
             for (int k = 0; k < kPauliPerThread; k++) {
-                if (tid + k * kBlock < tsz) {
+                if (FULL || tid + k * kBlock < tsz) {
                     nxt[k] = ld_nt<T>(&sv[base | off[k]]);
                 }
             }
         }
+        // one bucket's terms against the transform value (wr, wi)
+        auto bucket = [&](int t0, int t1, uint32_t e0, double wr, double wi) {
+            for (int t = t0; t < t1; t++) {
+                const PauliTileTerm tm = terms[t];
+                const uint32_t par = __popc(e0 & tm.zyl) + __popcll(cur & tm.zyo);
+                acc += pauli_flip(fma(tm.ci, wi, tm.cr * wr), par);
+            }
+        };
         for (int gi = 0; gi < ngroups; gi++) {
-            const uint32_t xl = groups[gi].xl;
+            const uint32_t xl = groups[gi].xl, used = groups[gi].used;
             if (xl == 0) {
                 double w[8];
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
                     const uint32_t e = tid + j * kBlock;
                     w[j] = 0;
-                    if (e < tsz) {
+                    if (FULL || e < tsz) {
                         const v2<T> a = tile[e];
                         w[j] = double(a.x) * double(a.x) + double(a.y) * double(a.y);
                     }
+                }
+                if (__popc(used) <= 3) {
+                    // each used transform value on its own: seven FMAs with uniform signs
+                    for (uint32_t u = used; u; u &= u - 1) {
+                        const int k = __ffs(u) - 1;
+                        double wk = w[0];
+#pragma unroll
+                        for (int j = 1; j < 8; j++) {
+                            wk = fma((__popc(j & k) & 1) ? -1.0 : 1.0, w[j], wk);
+                        }
+                        bucket(groups[gi].tb[k], groups[gi].tb[k + 1], tid, wk, 0.0);
+                    }
+                    continue;
                 }
 #pragma unroll
                 for (int h = 1; h < 8; h <<= 1) {
@@ -466,30 +489,40 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
                 }
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const int t1 = groups[gi].tb[k + 1];
-                    for (int t = groups[gi].tb[k]; t < t1; t++) {
-                        const PauliTileTerm tm = terms[t];
-                        const uint32_t par = __popc(tid & tm.zyl) + __popcll(cur & tm.zyo);
-                        acc += pauli_flip(tm.cr * w[k], par);
-                    }
+                    bucket(groups[gi].tb[k], groups[gi].tb[k + 1], tid, w[k], 0.0);
                 }
                 continue;
             }
             const int lb = __ffs(xl) - 1;
             const uint32_t npairs = tsz >> 1;
             const uint32_t e0 = pauli_insert0(tid, lb);
+            // the thread's pairs: low ends e0 ^ d_j, partners e0 ^ d_j ^ xl,
+            // d_j = pauli_insert0(256 j, lb) uniform (its bits are not e0's)
+            const uint32_t d1 = pauli_insert0(kBlock, lb), d2 = pauli_insert0(2 * kBlock, lb);
+            const char *tb8 = reinterpret_cast<const char *>(tile);
+            const uint32_t e0b = e0 * static_cast<uint32_t>(sizeof(v2<T>));
             double zr[kPauliPairs], zi[kPauliPairs];
 #pragma unroll
             for (int j = 0; j < kPauliPairs; j++) {
-                const uint32_t pe = tid + j * kBlock;
+                const uint32_t dj = ((j & 1) ? d1 : 0U) | ((j & 2) ? d2 : 0U);
                 zr[j] = zi[j] = 0;
-                if (pe < npairs) {
-                    const uint32_t e = pauli_insert0(pe, lb);
-                    const v2<T> a = tile[e], c = tile[e ^ xl];
+                if (FULL || tid + j * kBlock < npairs) {
+                    const v2<T> a = *reinterpret_cast<const v2<T> *>(tb8 + (e0b ^ (dj * sizeof(v2<T>))));
+                    const v2<T> c = *reinterpret_cast<const v2<T> *>(tb8 + (e0b ^ ((dj ^ xl) * sizeof(v2<T>))));
                     // z = conj(a) * c
                     zr[j] = double(a.x) * double(c.x) + double(a.y) * double(c.y);
                     zi[j] = double(a.x) * double(c.y) - double(a.y) * double(c.x);
                 }
+            }
+            if (__popc(used) <= 2) {
+                for (uint32_t u = used; u; u &= u - 1) {
+                    const int k = __ffs(u) - 1;
+                    const double s1 = (k & 1) ? -1.0 : 1.0, s2 = (k & 2) ? -1.0 : 1.0, s3 = s1 * s2;
+                    const double wr = fma(s3, zr[3], fma(s2, zr[2], fma(s1, zr[1], zr[0])));
+                    const double wi = fma(s3, zi[3], fma(s2, zi[2], fma(s1, zi[1], zi[0])));
+                    bucket(groups[gi].tb[k], groups[gi].tb[k + 1], e0, wr, wi);
+                }
+                continue;
             }
 #pragma unroll
             for (int h = 1; h < kPauliPairs; h <<= 1) {
@@ -506,12 +539,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             }
 #pragma unroll
             for (int k = 0; k < kPauliPairs; k++) {
-                const int t1 = groups[gi].tb[k + 1];
-                for (int t = groups[gi].tb[k]; t < t1; t++) {
-                    const PauliTileTerm tm = terms[t];
-                    const uint32_t par = __popc(e0 & tm.zyl) + __popcll(cur & tm.zyo);
-                    acc += pauli_flip(fma(tm.ci, zi[k], tm.cr * zr[k]), par);
-                }
+                bucket(groups[gi].tb[k], groups[gi].tb[k + 1], e0, zr[k], zi[k]);
             }
         }
     }

@@ -35,6 +35,9 @@
 #include <stdint.h>
 
 #include "kernels_gate.hpp"
+#include "kernels_tile_ip.hpp"
+
+#include <type_traits>
 
 namespace plamd {
 
@@ -72,8 +75,12 @@ __device__ __forceinline__ void tp_group(v2<T> *tile, const TpOp<T> &op, int k, 
         off[j] = op.off[j];
     }
     const bool diag = op.kind == 1;
-    for (int q = tid; q < ng; q += kTpLanes) {
-        uint32_t be = static_cast<uint32_t>(q);
+    // ng is a power of two: with fewer groups than lanes, lane t takes group
+    // t mod ng and lanes sharing a group write identical values, so the loop
+    // and every access are the same for all lanes (no divergent branch)
+#pragma unroll 1
+    for (int q0 = 0; q0 < ng; q0 += kTpLanes) {
+        uint32_t be = static_cast<uint32_t>((q0 + tid) & (ng - 1));
         for (int j = 0; j < op.nb; j++) {
             const int p = op.pos[j];
             be = ((be >> p) << (p + 1)) | (be & ((1U << p) - 1));
@@ -90,14 +97,21 @@ __device__ __forceinline__ void tp_group(v2<T> *tile, const TpOp<T> &op, int k, 
                 tile[be | off[j]] = cmul(op.mat[j * M + j], v[j]);
             }
         } else {
-#pragma unroll
+            // rows one at a time (not unrolled): each row's matrix entries are
+            // loaded where they are used instead of all M^2 being held
+#pragma unroll 1
             for (int r = 0; r < M; r++) {
                 v2<T> acc = cmul(op.mat[r * M], v[0]);
 #pragma unroll
                 for (int c = 1; c < M; c++) {
                     cfma(acc, op.mat[r * M + c], v[c]);
                 }
-                tile[be | off[r]] = acc;
+                uint32_t o = off[0];
+#pragma unroll
+                for (int j = 1; j < M; j++) {
+                    o = r == j ? off[j] : o;
+                }
+                tile[be | o] = acc;
             }
         }
     }
@@ -106,8 +120,28 @@ __device__ __forceinline__ void tp_group(v2<T> *tile, const TpOp<T> &op, int k, 
 template <class T>
 __device__ __forceinline__ void tp_parity(v2<T> *tile, const TpOp<T> &op, int k, int tid) {
     const dcx<T> p0 = op.mat[0], p1 = op.mat[1];
-    for (int e = tid; e < (1 << k); e += kTpLanes) {
+#pragma unroll 1
+    for (int e0 = 0; e0 < (1 << k); e0 += kTpLanes) { // 2^k >= 2^9: a whole number of wavefronts
+        const int e = e0 + tid;
         tile[e] = cmul((__popc(static_cast<uint32_t>(e) & op.mask) & 1) ? p1 : p0, tile[e]);
+    }
+}
+
+// Any op the register forms do not cover, applied to the tile staged in LDS
+// (the tile is in LDS, not in registers, while it runs).
+template <class T>
+__device__ __forceinline__ void tp_lds_op(v2<T> *tile, const TpOp<T> *opp, int k, int tid) {
+    const TpOp<T> &op = *opp;
+    if (op.kind == 2) {
+        tp_parity<T>(tile, op, k, tid);
+    } else if (op.M == 2) {
+        tp_group<T, 2>(tile, op, k, tid);
+    } else if (op.M == 4) {
+        tp_group<T, 4>(tile, op, k, tid);
+    } else if (op.M == 1) {
+        tp_group<T, 1>(tile, op, k, tid);
+    } else {
+        tp_group<T, 8>(tile, op, k, tid);
     }
 }
 
@@ -126,6 +160,33 @@ template <class T> __device__ __forceinline__ v2<T> tp_lane_xor(const v2<T> &v, 
     o.x = tp_bperm(v.x, src);
     o.y = tp_bperm(v.y, src);
     return o;
+}
+
+// A register-resident amplitude: two scalars rather than the native 2-vector
+// type, so each component is a register of its own that an update rewrites
+// in place (a 2-vector component update rebuilds the whole register tuple,
+// which costs copies at every merge of the interpreter's control flow).
+template <class T> struct TpAmp {
+    T x, y;
+    TpAmp() = default;
+    __device__ __forceinline__ TpAmp(const v2<T> &a) : x(a.x), y(a.y) {}
+    __device__ __forceinline__ operator v2<T>() const {
+        v2<T> o;
+        o.x = x;
+        o.y = y;
+        return o;
+    }
+};
+
+// A uniform matrix element read into SGPRs before a lane-dependent select
+// picks it.  Without the empty asm the compiler sinks the load into a branch
+// on the select's divergent condition, and that one divergent branch makes
+// the whole interpreter loop a structurized region (copies of the register
+// tile at every merge).
+template <class T> __device__ __forceinline__ dcx<T> tp_ld(const dcx<T> &a) {
+    dcx<T> r = a;
+    asm volatile("" : "+s"(r.r), "+s"(r.i));
+    return r;
 }
 
 enum TrCode : int {
@@ -169,10 +230,28 @@ template <class T> __device__ __forceinline__ v2<T> tp_mul2(dcx<T> a, v2<T> x, d
     return o;
 }
 
-// CHK: the op has register-bit controls (a uniform test per register group)
-template <class T, int KB, int J, bool CHK>
-__device__ __forceinline__ void tr_p1(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
-    const dcx<T> m00 = in.mat[0], m01 = in.mat[1], m10 = in.mat[2], m11 = in.mat[3];
+// In-place 2x2 update of the register pair (x, y) by m[0..3]: LC selects
+// the per-lane (VGPR) form of the matrix, otherwise it is uniform (SGPR).
+template <class T, bool LC>
+__device__ __forceinline__ void tr_pair2(TpAmp<T> &x, TpAmp<T> &y, const dcx<T> (&m)[16]) {
+    if constexpr (std::is_same_v<T, double>) {
+        if constexpr (LC) {
+            ip_pair_v(x.x, x.y, y.x, y.y, m[0].r, m[0].i, m[1].r, m[1].i, m[2].r, m[2].i, m[3].r, m[3].i);
+        } else {
+            ip_pair_s(x.x, x.y, y.x, y.y, m[0].r, m[0].i, m[1].r, m[1].i, m[2].r, m[2].i, m[3].r, m[3].i);
+        }
+    } else {
+        const v2<T> a = x, c = y;
+        x = tp_mul2(m[0], a, m[1], c);
+        y = tp_mul2(m[2], a, m[3], c);
+    }
+}
+
+// Group ops whose member bits are register bits.  m: the op's matrix, the
+// identity on lanes whose own condition fails.  CHK: the op has register-bit
+// controls (a uniform test per register group).
+template <class T, int KB, int J, bool CHK, bool LC>
+__device__ __forceinline__ void tr_p1(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
         if ((r >> J) & 1) {
@@ -181,17 +260,13 @@ __device__ __forceinline__ void tr_p1(v2<T> (&v)[1 << KB], const TrInst<T> &in) 
         if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
             continue;
         }
-        constexpr int b = 1 << J;
-        const v2<T> a = v[r], c = v[r | b];
-        v[r] = tp_mul2(m00, a, m01, c);
-        v[r | b] = tp_mul2(m10, a, m11, c);
+        tr_pair2<T, LC>(v[r], v[r | (1 << J)], m);
     }
 }
 
 // members r | 1 << JA and r | 1 << JB (bits JA, JB of r clear)
-template <class T, int KB, int JA, int JB, bool CHK>
-__device__ __forceinline__ void tr_p2(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
-    const dcx<T> m00 = in.mat[0], m01 = in.mat[1], m10 = in.mat[2], m11 = in.mat[3];
+template <class T, int KB, int JA, int JB, bool CHK, bool LC>
+__device__ __forceinline__ void tr_p2(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
         if (((r >> JA) & 1) || ((r >> JB) & 1)) {
@@ -200,16 +275,13 @@ __device__ __forceinline__ void tr_p2(v2<T> (&v)[1 << KB], const TrInst<T> &in) 
         if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
             continue;
         }
-        constexpr int a_ = 1 << JA, b_ = 1 << JB;
-        const v2<T> a = v[r | a_], c = v[r | b_];
-        v[r | a_] = tp_mul2(m00, a, m01, c);
-        v[r | b_] = tp_mul2(m10, a, m11, c);
+        tr_pair2<T, LC>(v[r | (1 << JA)], v[r | (1 << JB)], m);
     }
 }
 
 // members r, r | 1 << JA, r | 1 << JB, r | both
-template <class T, int KB, int JA, int JB, bool CHK>
-__device__ __forceinline__ void tr_p4(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+template <class T, int KB, int JA, int JB, bool CHK, bool LC>
+__device__ __forceinline__ void tr_p4(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
         if (((r >> JA) & 1) || ((r >> JB) & 1)) {
@@ -223,10 +295,10 @@ __device__ __forceinline__ void tr_p4(v2<T> (&v)[1 << KB], const TrInst<T> &in) 
         v2<T> y[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            y[i] = tp_mul(in.mat[4 * i], x[0]);
+            y[i] = tp_mul(m[4 * i], x[0]);
 #pragma unroll
             for (int c = 1; c < 4; c++) {
-                tp_mac(y[i], in.mat[4 * i + c], x[c]);
+                tp_mac(y[i], m[4 * i + c], x[c]);
             }
         }
         v[r] = y[0];
@@ -240,7 +312,7 @@ __device__ __forceinline__ void tr_p4(v2<T> (&v)[1 << KB], const TrInst<T> &in) 
 // its compile-time register indices.
 // diagonal op: v[r] *= pt[c_r], c_r = the phase-index bits register r feeds
 template <class T, int KB, bool CHK, int NR, int J0, int J1>
-__device__ __forceinline__ void tr_diag(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+__device__ __forceinline__ void tr_diag(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
         if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
@@ -253,12 +325,16 @@ __device__ __forceinline__ void tr_diag(v2<T> (&v)[1 << KB], const TrInst<T> &in
         if constexpr (NR >= 2) {
             c |= ((r >> J1) & 1) << 1;
         }
-        v[r] = tp_mul(pt[c], v[r]);
+        if constexpr (std::is_same_v<T, double>) {
+            ip_scale(v[r].x, v[r].y, pt[c].r, pt[c].i);
+        } else {
+            v[r] = tp_mul(pt[c], v[r]);
+        }
     }
 }
 
 template <class T, int KB, bool CHK, int J = 0>
-__device__ __forceinline__ void tr_diag1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+__device__ __forceinline__ void tr_diag1_at(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
     if constexpr (J < KB) {
         if (in.ja == J) {
             tr_diag<T, KB, CHK, 1, J, 0>(v, in, pt);
@@ -268,7 +344,7 @@ __device__ __forceinline__ void tr_diag1_at(v2<T> (&v)[1 << KB], const TrInst<T>
 }
 
 template <class T, int KB, bool CHK, int JA = 0, int JB = 1>
-__device__ __forceinline__ void tr_diag2_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
+__device__ __forceinline__ void tr_diag2_at(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&pt)[4]) {
     if constexpr (JB < KB) {
         if (in.ja == JA && in.jb == JB) {
             tr_diag<T, KB, CHK, 2, JA, JB>(v, in, pt);
@@ -281,122 +357,157 @@ __device__ __forceinline__ void tr_diag2_at(v2<T> (&v)[1 << KB], const TrInst<T>
     }
 }
 
-template <class T, int KB, bool CHK, int J = 0>
-__device__ __forceinline__ void tr_p1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+template <class T, int KB, bool CHK, bool LC, int J = 0>
+__device__ __forceinline__ void tr_p1_at(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
     if constexpr (J < KB) {
         if (in.ja == J) {
-            tr_p1<T, KB, J, CHK>(v, in);
+            tr_p1<T, KB, J, CHK, LC>(v, in, m);
         }
-        tr_p1_at<T, KB, CHK, J + 1>(v, in);
+        tr_p1_at<T, KB, CHK, LC, J + 1>(v, in, m);
     }
 }
 
-template <class T, int KB, bool P4, bool CHK, int JA = 0, int JB = 1>
-__device__ __forceinline__ void tr_pair_at(v2<T> (&v)[1 << KB], const TrInst<T> &in) {
+template <class T, int KB, bool P4, bool CHK, bool LC, int JA = 0, int JB = 1>
+__device__ __forceinline__ void tr_pair_at(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
     if constexpr (JB < KB) {
         if (in.ja == JA && in.jb == JB) {
             if constexpr (P4) {
-                tr_p4<T, KB, JA, JB, CHK>(v, in);
+                tr_p4<T, KB, JA, JB, CHK, LC>(v, in, m);
             } else {
-                tr_p2<T, KB, JA, JB, CHK>(v, in);
+                tr_p2<T, KB, JA, JB, CHK, LC>(v, in, m);
             }
         }
         if constexpr (JA + 1 < JB) {
-            tr_pair_at<T, KB, P4, CHK, JA + 1, JB>(v, in);
+            tr_pair_at<T, KB, P4, CHK, LC, JA + 1, JB>(v, in, m);
         } else {
-            tr_pair_at<T, KB, P4, CHK, 0, JB + 1>(v, in);
+            tr_pair_at<T, KB, P4, CHK, LC, 0, JB + 1>(v, in, m);
+        }
+    }
+}
+
+// register-member group op with uniform (LC false) or per-lane matrix
+template <class T, int KB, bool LC>
+__device__ __forceinline__ void tr_pgroup(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&m)[16]) {
+    const bool chk = in.rmask != 0;
+    if (in.code == kTrP1) {
+        if (chk) {
+            tr_p1_at<T, KB, true, LC>(v, in, m);
+        } else {
+            tr_p1_at<T, KB, false, LC>(v, in, m);
+        }
+    } else if (in.code == kTrP2) {
+        if (chk) {
+            tr_pair_at<T, KB, false, true, LC>(v, in, m);
+        } else {
+            tr_pair_at<T, KB, false, false, LC>(v, in, m);
+        }
+    } else if constexpr (!LC) { // kTrP4 carries no lane condition (the host sends those through LDS)
+        if (chk) {
+            tr_pair_at<T, KB, true, true, false>(v, in, m);
+        } else {
+            tr_pair_at<T, KB, true, false, false>(v, in, m);
         }
     }
 }
 
 // one lane member bit (lx) and one register member bit J: per register pair
-// a lane exchanges both amplitudes with lane ^ lx and forms its own two
-// GEN: the op has controls (register-bit test per pair, lane predicate `on`)
-template <class T, int KB, int J, bool GEN>
-__device__ __forceinline__ void tr_mix1(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
-                                        int src, bool on) {
+// a lane exchanges both amplitudes with lane ^ lx and forms its own two.
+// CHK: the op has register-bit controls (a uniform test per pair); lanes
+// whose own condition fails carry identity weights
+template <class T, int KB, int J, bool CHK>
+__device__ __forceinline__ void tr_mix1(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
+                                        int src) {
 #pragma unroll
     for (int r = 0; r < (1 << KB); r++) {
         if ((r >> J) & 1) {
             continue;
         }
-        if (GEN && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
             continue;
         }
         constexpr int b = 1 << J;
-        const v2<T> a0 = v[r], a1 = v[r | b];
-        const v2<T> p0 = tp_lane_xor<T>(a0, src), p1 = tp_lane_xor<T>(a1, src);
-        v2<T> y0 = tp_mul(w[0], a0), y1 = tp_mul(w[4], a0);
-        tp_mac(y0, w[1], a1);
-        tp_mac(y1, w[5], a1);
-        tp_mac(y0, w[2], p0);
-        tp_mac(y1, w[6], p0);
-        tp_mac(y0, w[3], p1);
-        tp_mac(y1, w[7], p1);
-        if (GEN) {
-            v[r].x = on ? y0.x : a0.x;
-            v[r].y = on ? y0.y : a0.y;
-            v[r | b].x = on ? y1.x : a1.x;
-            v[r | b].y = on ? y1.y : a1.y;
+        const v2<T> p0 = tp_lane_xor<T>(v[r], src), p1 = tp_lane_xor<T>(v[r | b], src);
+        if constexpr (std::is_same_v<T, double>) {
+            const double ww[16] = {w[0].r, w[0].i, w[1].r, w[1].i, w[2].r, w[2].i, w[3].r, w[3].i,
+                                   w[4].r, w[4].i, w[5].r, w[5].i, w[6].r, w[6].i, w[7].r, w[7].i};
+            ip_mix1(v[r].x, v[r].y, v[r | b].x, v[r | b].y, p0.x, p0.y, p1.x, p1.y, ww);
         } else {
+            const v2<T> a0 = v[r], a1 = v[r | b];
+            v2<T> y0 = tp_mul(w[0], a0), y1 = tp_mul(w[4], a0);
+            tp_mac(y0, w[1], a1);
+            tp_mac(y1, w[5], a1);
+            tp_mac(y0, w[2], p0);
+            tp_mac(y1, w[6], p0);
+            tp_mac(y0, w[3], p1);
+            tp_mac(y1, w[7], p1);
             v[r] = y0;
             v[r | b] = y1;
         }
     }
 }
 
-template <class T, int KB, bool GEN, int J = 0>
-__device__ __forceinline__ void tr_mix1_at(v2<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
-                                           int src, bool on) {
+template <class T, int KB, bool CHK, int J = 0>
+__device__ __forceinline__ void tr_mix1_at(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[8],
+                                           int src) {
     if constexpr (J < KB) {
         if (in.ja == J) {
-            tr_mix1<T, KB, J, GEN>(v, in, w, src, on);
+            tr_mix1<T, KB, J, CHK>(v, in, w, src);
         }
-        tr_mix1_at<T, KB, GEN, J + 1>(v, in, w, src, on);
+        tr_mix1_at<T, KB, CHK, J + 1>(v, in, w, src);
     }
 }
 
-// KB = k - 6: the number of local bits held in registers.  One wavefront
-// per tile; `lds_bytes` of dynamic LDS only when the pass has lds ops.
+// 2x2 op on a lane bit: each register takes its partner from lane ^ lx.
+// CHK: register-bit controls; lanes whose own condition fails carry
+// identity weights (a = 1, b = 0), so every lane runs the same code
+template <class T, int KB, bool CHK>
+__device__ __forceinline__ void tr_lane(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, dcx<T> a, dcx<T> b, int src) {
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        const v2<T> o = tp_lane_xor<T>(v[r], src);
+        if constexpr (std::is_same_v<T, double>) {
+            ip_lane(v[r].x, v[r].y, o.x, o.y, a.r, a.i, b.r, b.i);
+        } else {
+            v[r] = tp_mul2(a, v[r], b, o);
+        }
+    }
+}
+
+// two lane member bits: three exchanges per register (same conventions)
+template <class T, int KB, bool CHK>
+__device__ __forceinline__ void tr_mix2(TpAmp<T> (&v)[1 << KB], const TrInst<T> &in, const dcx<T> (&w)[4], int lane,
+                                        int l1, int l2) {
+#pragma unroll
+    for (int r = 0; r < (1 << KB); r++) {
+        if (CHK && (static_cast<uint32_t>(r) & in.rmask) != in.rval) {
+            continue;
+        }
+        const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
+        const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
+        const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
+        if constexpr (std::is_same_v<T, double>) {
+            const double ww[8] = {w[0].r, w[0].i, w[1].r, w[1].i, w[2].r, w[2].i, w[3].r, w[3].i};
+            ip_mix2(v[r].x, v[r].y, s1.x, s1.y, s2.x, s2.y, s3.x, s3.y, ww);
+        } else {
+            v2<T> y = tp_mul(w[0], v[r]);
+            tp_mac(y, w[1], s1);
+            tp_mac(y, w[2], s2);
+            tp_mac(y, w[3], s3);
+            v[r] = y;
+        }
+    }
+}
+
+// The program of a pass applied to one tile held in registers (v[r]: the
+// amplitude of local index lane | r << 6).  lds: the workgroup's LDS tile,
+// used only by kTrLds operations.
 template <class T, int KB>
-__global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, uint64_t ntiles,
-                                                        const TpGeom g, const TrInst<T> *__restrict__ prog,
-                                                        int ninst, const TpOp<T> *__restrict__ lops) {
-    extern __shared__ unsigned char smem_raw[];
-    v2<T> *lds = reinterpret_cast<v2<T> *>(smem_raw);
+__device__ __forceinline__ void tp_run(TpAmp<T> (&v)[1 << KB], const TrInst<T> *__restrict__ prog, int ninst,
+                                       const TpOp<T> *__restrict__ lops, v2<T> *lds, int lane) {
     constexpr int R = 1 << KB;
-    const uint64_t t = xcd_block();
-    if (t >= ntiles) {
-        return;
-    }
-    const int lane = threadIdx.x;
-    const uint64_t base = expand(t << g.low, g.outer);
-    // HBM offsets of the lane's local bits 0..5 and of each register's bits
-    uint64_t loff = static_cast<uint64_t>(lane) & ((uint64_t{1} << g.low) - 1);
-    const int nlane_hi = 6 - g.low;
-    for (int j = 0; j < nlane_hi; j++) {
-        if ((lane >> (g.low + j)) & 1) {
-            loff |= uint64_t{1} << g.hb[j];
-        }
-    }
-    v2<T> *const p = sv + (base | loff);
-    uint64_t roff[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int j = 0; j < KB; j++) {
-            if ((r >> j) & 1) {
-                o |= uint64_t{1} << g.hb[nlane_hi + j];
-            }
-        }
-        roff[r] = o;
-    }
-    v2<T> v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        v[r] = ld_nt<T>(p + roff[r]);
-    }
     const int k = 6 + KB;
     for (int pc = 0; pc < ninst; pc++) {
         const TrInst<T> &in = prog[pc];
@@ -411,7 +522,7 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
             if (in.tcb[1] >= 0) {
                 ct |= ((lane >> in.tcb[1]) & 1) << 1;
             }
-            const dcx<T> d0 = in.mat[0], d1 = in.mat[1], d2 = in.mat[2], d3 = in.mat[3];
+            const dcx<T> d0 = tp_ld(in.mat[0]), d1 = tp_ld(in.mat[1]), d2 = tp_ld(in.mat[2]), d3 = tp_ld(in.mat[3]);
             const int nr = in.nr;
             dcx<T> pt[4];
 #pragma unroll
@@ -450,39 +561,32 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
             }
         } else if (code == kTrParity) {
             const bool odd = __popc(static_cast<uint32_t>(lane) & in.tmask) & 1;
-            const dcx<T> e0 = in.mat[0], e1 = in.mat[1];
+            const dcx<T> e0 = tp_ld(in.mat[0]), e1 = tp_ld(in.mat[1]);
             const dcx<T> p0{odd ? e1.r : e0.r, odd ? e1.i : e0.i};
             const dcx<T> p1{odd ? e0.r : e1.r, odd ? e0.i : e1.i};
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const bool rodd = __popc(static_cast<uint32_t>(r) & in.rmask) & 1;
-                v[r] = tp_mul(dcx<T>{rodd ? p1.r : p0.r, rodd ? p1.i : p0.i}, v[r]);
+                const dcx<T> ph{rodd ? p1.r : p0.r, rodd ? p1.i : p0.i};
+                if constexpr (std::is_same_v<T, double>) {
+                    ip_scale(v[r].x, v[r].y, ph.r, ph.i);
+                } else {
+                    v[r] = tp_mul(ph, v[r]);
+                }
             }
         } else if (code == kTrLane) {
             // partner amplitude from lane ^ lx; every lane takes part in the exchange
             const bool hi = (static_cast<uint32_t>(lane) & in.lx) != 0;
             const bool on = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
-            const dcx<T> m0 = in.mat[0], m1 = in.mat[1], m2 = in.mat[2], m3 = in.mat[3];
-            const dcx<T> a{hi ? m3.r : m0.r, hi ? m3.i : m0.i}; // own amplitude
-            const dcx<T> b{hi ? m2.r : m1.r, hi ? m2.i : m1.i}; // partner amplitude
+            const dcx<T> m0 = tp_ld(in.mat[0]), m1 = tp_ld(in.mat[1]), m2 = tp_ld(in.mat[2]), m3 = tp_ld(in.mat[3]);
+            // own and partner weights; identity where the lane condition fails
+            const dcx<T> a{on ? (hi ? m3.r : m0.r) : T(1), on ? (hi ? m3.i : m0.i) : T(0)};
+            const dcx<T> b{on ? (hi ? m2.r : m1.r) : T(0), on ? (hi ? m2.i : m1.i) : T(0)};
             const int src = lane ^ static_cast<int>(in.lx);
-            if (in.rmask == 0 && in.tmask == 0) { // no controls
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const v2<T> o = tp_lane_xor<T>(v[r], src);
-                    v[r] = tp_mul2(a, v[r], b, o);
-                }
+            if (in.rmask == 0) {
+                tr_lane<T, KB, false>(v, in, a, b, src);
             } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
-                        continue;
-                    }
-                    const v2<T> o = tp_lane_xor<T>(v[r], src);
-                    const v2<T> y = tp_mul2(a, v[r], b, o);
-                    v[r].x = on ? y.x : v[r].x; // componentwise: no aggregate select
-                    v[r].y = on ? y.y : v[r].y;
-                }
+                tr_lane<T, KB, true>(v, in, a, b, src);
             }
         } else if (code == kTrMix1) {
             const bool hi = (static_cast<uint32_t>(lane) & in.lx) != 0;
@@ -490,14 +594,16 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
             dcx<T> w[8];
 #pragma unroll
             for (int c = 0; c < 8; c++) {
-                const dcx<T> lo_ = in.mat[c], hi_ = in.mat[8 + c];
-                w[c] = dcx<T>{hi ? hi_.r : lo_.r, hi ? hi_.i : lo_.i};
+                const dcx<T> lo_ = tp_ld(in.mat[c]), hi_ = tp_ld(in.mat[8 + c]);
+                // identity rows (own amplitude, weight 1) where the lane condition fails
+                const T idr = (c == 0 || c == 5) ? T(1) : T(0);
+                w[c] = dcx<T>{on ? (hi ? hi_.r : lo_.r) : idr, on ? (hi ? hi_.i : lo_.i) : T(0)};
             }
             const int src = lane ^ static_cast<int>(in.lx);
-            if (in.rmask != 0 || in.tmask != 0) {
-                tr_mix1_at<T, KB, true>(v, in, w, src, on);
+            if (in.rmask != 0) {
+                tr_mix1_at<T, KB, true>(v, in, w, src);
             } else {
-                tr_mix1_at<T, KB, false>(v, in, w, src, on);
+                tr_mix1_at<T, KB, false>(v, in, w, src);
             }
         } else if (code == kTrMix2) {
             // two lane member bits: three exchanges per register
@@ -506,93 +612,86 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
             dcx<T> w[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
-                const dcx<T> e0 = in.mat[d], e1 = in.mat[4 + d], e2 = in.mat[8 + d], e3 = in.mat[12 + d];
+                const dcx<T> e0 = tp_ld(in.mat[d]), e1 = tp_ld(in.mat[4 + d]), e2 = tp_ld(in.mat[8 + d]),
+                             e3 = tp_ld(in.mat[12 + d]);
                 const bool b0 = o & 1, b1 = o & 2;
                 const T lr = b0 ? e1.r : e0.r, li = b0 ? e1.i : e0.i;
                 const T hr = b0 ? e3.r : e2.r, hi = b0 ? e3.i : e2.i;
-                w[d] = dcx<T>{b1 ? hr : lr, b1 ? hi : li};
+                w[d] = dcx<T>{on ? (b1 ? hr : lr) : (d == 0 ? T(1) : T(0)), on ? (b1 ? hi : li) : T(0)};
             }
             const int l1 = static_cast<int>(in.lx), l2 = static_cast<int>(in.lx2);
-            if (in.rmask == 0 && in.tmask == 0) { // no controls
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
-                    const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
-                    const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
-                    v2<T> y = tp_mul(w[0], v[r]);
-                    tp_mac(y, w[1], s1);
-                    tp_mac(y, w[2], s2);
-                    tp_mac(y, w[3], s3);
-                    v[r] = y;
-                }
+            if (in.rmask == 0) {
+                tr_mix2<T, KB, false>(v, in, w, lane, l1, l2);
             } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if ((static_cast<uint32_t>(r) & in.rmask) != in.rval) {
-                        continue;
-                    }
-                    const v2<T> s1 = tp_lane_xor<T>(v[r], lane ^ l1);
-                    const v2<T> s2 = tp_lane_xor<T>(v[r], lane ^ l2);
-                    const v2<T> s3 = tp_lane_xor<T>(v[r], lane ^ l1 ^ l2);
-                    v2<T> y = tp_mul(w[0], v[r]);
-                    tp_mac(y, w[1], s1);
-                    tp_mac(y, w[2], s2);
-                    tp_mac(y, w[3], s3);
-                    v[r].x = on ? y.x : v[r].x;
-                    v[r].y = on ? y.y : v[r].y;
-                }
+                tr_mix2<T, KB, true>(v, in, w, lane, l1, l2);
             }
         } else if (code == kTrLds) {
+            v2<T> *const lp = lds + lane; // register r at lp[64 r]: immediate offsets
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                lds[lane | (r << 6)] = v[r];
+                lp[r * kTpLanes] = v[r];
             }
             __syncthreads();
-            const TpOp<T> &op = lops[in.lds_op];
-            if (op.kind == 2) {
-                tp_parity<T>(lds, op, k, lane);
-            } else if (op.M == 2) {
-                tp_group<T, 2>(lds, op, k, lane);
-            } else if (op.M == 4) {
-                tp_group<T, 4>(lds, op, k, lane);
-            } else if (op.M == 1) {
-                tp_group<T, 1>(lds, op, k, lane);
-            } else {
-                tp_group<T, 8>(lds, op, k, lane);
-            }
+            tp_lds_op<T>(lds, lops + in.lds_op, k, lane);
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                v[r] = lds[lane | (r << 6)];
+                v[r] = lp[r * kTpLanes];
             }
             __syncthreads();
-        } else if ((static_cast<uint32_t>(lane) & in.tmask) == in.tval) {
-            const bool chk = in.rmask != 0;
-            if (code == kTrP1) {
-                if (chk) {
-                    tr_p1_at<T, KB, true>(v, in);
-                } else {
-                    tr_p1_at<T, KB, false>(v, in);
+        } else { // kTrP1, kTrP2, kTrP4
+            dcx<T> m[16];
+            if (in.tmask == 0) {
+#pragma unroll
+                for (int b = 0; b < 16; b++) {
+                    m[b] = in.mat[b];
                 }
-            } else if (code == kTrP2) {
-                if (chk) {
-                    tr_pair_at<T, KB, false, true>(v, in);
-                } else {
-                    tr_pair_at<T, KB, false, false>(v, in);
-                }
-            } else if (chk) {
-                tr_pair_at<T, KB, true, true>(v, in);
+                tr_pgroup<T, KB, false>(v, in, m);
             } else {
-                tr_pair_at<T, KB, true, false>(v, in);
+                // kTrP1 / kTrP2: lanes whose own condition fails apply the identity
+                const bool on = (static_cast<uint32_t>(lane) & in.tmask) == in.tval;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const T id = (b == 0 || b == 3) ? T(1) : T(0);
+                    const dcx<T> mb = tp_ld(in.mat[b]);
+                    m[b] = dcx<T>{on ? mb.r : id, on ? mb.i : T(0)};
+                }
+                tr_pgroup<T, KB, true>(v, in, m);
             }
         }
     }
-    // store offsets recomputed from the kernel arguments: the empty asm keeps
-    // the compiler from holding the load-time offsets in registers through
-    // the whole program
-    uint64_t soff[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
+}
+
+// KB = k - 6: the number of local bits held in registers.  One wavefront
+// per tile, one tile per wavefront: the hardware launches a new wavefront as
+// soon as one retires.  (Measured at 30 qubits, complex128: 7.1 ms per pass;
+// persistent wavefronts that prefetch their next tile into registers, at
+// two per SIMD, took 7.9 ms and lower DRAM efficiency; forcing four per
+// SIMD instead of three gained nothing.)  `lds_bytes` of dynamic LDS only
+// when the pass has lds ops.
+template <class T, int KB>
+__global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, uint64_t ntiles,
+                                                        const TpGeom g, const TrInst<T> *__restrict__ prog,
+                                                        int ninst, const TpOp<T> *__restrict__ lops) {
+    extern __shared__ unsigned char smem_raw[];
+    v2<T> *lds = reinterpret_cast<v2<T> *>(smem_raw);
+    constexpr int R = 1 << KB;
+    const uint64_t t = xcd_block();
+    if (t >= ntiles) {
+        return;
+    }
+    const int lane = threadIdx.x;
+    // HBM offsets of the lane's local bits 0..5 and of each register's bits
+    uint64_t loff = static_cast<uint64_t>(lane) & ((uint64_t{1} << g.low) - 1);
+    const int nlane_hi = 6 - g.low;
+    for (int j = 0; j < nlane_hi; j++) {
+        if ((lane >> (g.low + j)) & 1) {
+            loff |= uint64_t{1} << g.hb[j];
+        }
+    }
+    // a register's offset: uniform, recomputed where used (the empty asm keeps
+    // the compiler from holding all of them through the program)
+    auto roff = [&](int r) {
         uint64_t o = 0;
 #pragma unroll
         for (int j = 0; j < KB; j++) {
@@ -602,12 +701,18 @@ __global__ __launch_bounds__(kTpLanes) void k_tile_pass(v2<T> *__restrict__ sv, 
                 o |= uint64_t{1} << hbj;
             }
         }
-        soff[r] = o;
-    }
+        return o;
+    };
+    v2<T> *const p = sv + (expand(t << g.low, g.outer) | loff);
+    TpAmp<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        st_nt<T>(p + soff[r], v[r]);
+        v[r] = ld_nt<T>(p + roff(r));
+    }
+    tp_run<T, KB>(v, prog, ninst, lops, lds, lane);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        st_nt<T>(p + roff(r), v[r]);
     }
 }
-
 } // namespace plamd

@@ -422,6 +422,7 @@ template <class T> struct Engine {
     // is fastest with 2^10-amplitude tiles and 3 low bits, complex64 at 33
     // qubits with 2^11 and 4 (register pressure bounds larger tiles)
     static constexpr int kTileDefaultBits = std::is_same_v<T, double> ? 10 : 11;
+    static constexpr int kTileMaxBitsC128 = 11;
     static constexpr int kTileDefaultLow = std::is_same_v<T, double> ? 3 : 4;
 
 
@@ -453,6 +454,10 @@ template <class T> struct Engine {
             low = std::min(kTileDefaultLow, k - 6);
         }
         const auto passes = tile_schedule(n(), t, k, low);
+        if (std::is_same_v<T, double> && k > kTileMaxBitsC128) {
+            throw InvalidArg("complex128 tiles hold at most 2^" + std::to_string(kTileMaxBitsC128) +
+                             " amplitudes (a wavefront's registers)");
+        }
         int dev_id = 0, lds_max = 0;
         HIP_OK(hipGetDevice(&dev_id));
         HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id));
@@ -672,6 +677,7 @@ template <class T> struct Engine {
                 }
                 for (int k = 0; k < nb; k++) {
                     gr.tb[k] = static_cast<int>(ttab.size());
+                    gr.used |= bucket[k].empty() ? 0U : 1U << k;
                     ttab.insert(ttab.end(), bucket[k].begin(), bucket[k].end());
                 }
                 gr.tb[nb] = static_cast<int>(ttab.size());
@@ -717,9 +723,15 @@ template <class T> struct Engine {
             const int nS = g.nlow + g.nhigh;
             const uint64_t ntiles = len() >> nS;
             const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(ntiles, kReduceBlocks));
-            hipLaunchKernelGGL((k_pauli_tile<T>), dim3(grid), dim3(kBlock), (size_t{1} << nS) * sizeof(V),
-                               h->stream, sv, ntiles, g, static_cast<int>(gspan[pi].second),
-                               dgroups + gspan[pi].first, dtterms, part + total);
+            const size_t lds = (size_t{1} << nS) * sizeof(V);
+            const int ng = static_cast<int>(gspan[pi].second);
+            if (nS == 3 + kPauliTileHigh) { // full tiles (every state of >= 11 qubits)
+                hipLaunchKernelGGL((k_pauli_tile<T, true>), dim3(grid), dim3(kBlock), lds, h->stream, sv, ntiles, g,
+                                   ng, dgroups + gspan[pi].first, dtterms, part + total);
+            } else {
+                hipLaunchKernelGGL((k_pauli_tile<T, false>), dim3(grid), dim3(kBlock), lds, h->stream, sv, ntiles, g,
+                                   ng, dgroups + gspan[pi].first, dtterms, part + total);
+            }
             HIP_OK(hipGetLastError());
             total += grid;
         }

@@ -346,7 +346,8 @@ void tile_program(int n, const TapeView &t, const TilePass &ps, int low, const s
                         __builtin_popcountll(g.off[1]) == 1;
         const bool p4 = clean && g.M == 4 && g.off[0] == 0 && nx == 2 && __builtin_popcountll(g.off[1]) == 1 &&
                         __builtin_popcountll(g.off[2]) == 1 && g.off[3] == (g.off[1] | g.off[2]);
-        if ((p1 || p2 || p4) && (xo & ~reg_bits) == 0) { // members in registers
+        // members in registers (a 4x4 op conditioned on lane bits goes through LDS)
+        if ((p1 || p2 || (p4 && in.tmask == 0)) && (xo & ~reg_bits) == 0) {
             std::vector<std::complex<T>> m = g.mat;
             if (p1) {
                 in.code = kTrP1;

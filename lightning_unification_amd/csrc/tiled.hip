@@ -10,6 +10,7 @@
 namespace plamd {
 
 using TileT = std::conditional_t<PL_TILE_DOUBLE, double, float>;
+static_assert(!(PL_TILE_DOUBLE && PL_TILE_KB > 5), "complex128 tiles hold at most 2^11 amplitudes");
 
 template <>
 hipError_t launch_tile_pass_kb<TileT, PL_TILE_KB>(v2<TileT> *sv, uint64_t ntiles, const TpGeom &g,

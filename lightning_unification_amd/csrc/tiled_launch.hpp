@@ -7,12 +7,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels_tile.hpp"
 
 namespace plamd {
 
-// One pass over every tile of 2^(6 + KB) amplitudes (KB = 3..6): one
-// wavefront per tile, LDS only when the pass has LDS operations.
+// One pass over every tile of 2^(6 + KB) amplitudes (KB = 3..6 for
+// complex64, 3..5 for complex128: 64 registers of complex128 per lane do
+// not fit a wavefront's register file): one wavefront per tile, LDS only
+// when the pass has LDS operations.
 template <class T, int KB>
 hipError_t launch_tile_pass_kb(v2<T> *sv, uint64_t ntiles, const TpGeom &g, const TrInst<T> *prog, int ninst,
                                const TpOp<T> *lops, bool uses_lds, hipStream_t stream);
@@ -29,7 +33,6 @@ PL_TILE_DECLARE(float, 6)
 PL_TILE_DECLARE(double, 3)
 PL_TILE_DECLARE(double, 4)
 PL_TILE_DECLARE(double, 5)
-PL_TILE_DECLARE(double, 6)
 #undef PL_TILE_DECLARE
 
 template <class T>
@@ -39,7 +42,12 @@ inline hipError_t launch_tile_pass(int kb, v2<T> *sv, uint64_t ntiles, const TpG
     case 3: return launch_tile_pass_kb<T, 3>(sv, ntiles, g, prog, ninst, lops, uses_lds, stream);
     case 4: return launch_tile_pass_kb<T, 4>(sv, ntiles, g, prog, ninst, lops, uses_lds, stream);
     case 5: return launch_tile_pass_kb<T, 5>(sv, ntiles, g, prog, ninst, lops, uses_lds, stream);
-    default: return launch_tile_pass_kb<T, 6>(sv, ntiles, g, prog, ninst, lops, uses_lds, stream);
+    default:
+        if constexpr (std::is_same_v<T, float>) {
+            return launch_tile_pass_kb<T, 6>(sv, ntiles, g, prog, ninst, lops, uses_lds, stream);
+        } else {
+            return hipErrorInvalidValue; // rejected by the caller before any launch
+        }
     }
 }
 

@@ -26,6 +26,12 @@ def test_tiled_matches_oracle(oracle, dtype, n, k, low):
     exp = oracle.apply_tape(psi.astype(np.complex128), without_prep(tape))
     print(f"tiled n={n} k={k} low={low} {np.dtype(dtype).name}", flush=True)
     with capi.DeviceStateVector.from_array(psi) as sv:
+        if dtype == np.complex128 and k > 11:
+            # a 2^12-amplitude complex128 tile does not fit a wavefront's registers
+            with pytest.raises(ValueError):
+                sv.apply_tape_tiled(tape, k, low)
+            assert rel_err(sv.state(), psi) == 0  # nothing applied
+            return
         npass = sv.apply_tape_tiled(tape, k, low)
         got = sv.state()
     assert npass == len(capi.tile_plan(n, tape, k, low))
@@ -33,12 +39,14 @@ def test_tiled_matches_oracle(oracle, dtype, n, k, low):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("k,low", [(9, 1), (12, 6), (11, 4)])
+@pytest.mark.parametrize("k,low", [(9, 1), (12, 6), (11, 4), (10, 3)])
 def test_tiled_matches_per_gate_path_20q(dtype, k, low):
     from lightning_unification_amd import capi
     n = 20
     tape = random_layers(n, 50, seed=1002)
     print(f"tiled 20q k={k} low={low} {np.dtype(dtype).name}", flush=True)
+    if dtype == np.complex128 and k > 11:
+        pytest.skip("complex128 tiles hold at most 2^11 amplitudes (covered by the oracle test)")
     with capi.DeviceStateVector(n, dtype) as a, capi.DeviceStateVector(n, dtype) as b:
         a.apply_tape(tape)
         npass = b.apply_tape_tiled(tape, k, low)
