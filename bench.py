@@ -529,9 +529,10 @@ def bench_precision(n_big=33, n_ref=30):
     gate_bytes = sum(2 * touched_fraction(op[0]) * S for op in tape.ops)
     block_bytes = nb * 2 * S  # every block reads and writes the state once
     # a 32x32 complex block per group of 32 amplitudes: 8 * 32 * 32 FLOP of
-    # complex arithmetic, executed as three real 32x32 products (Gauss)
+    # complex arithmetic, executed as four real 32x32 products (the real
+    # 64x64 form [[Ur, -Ui], [Ui, Ur]]), so the matrix cores execute as many
     flops = nb * (1 << (n_big - 5)) * 8 * 32 * 32
-    mfma_flops = nb * (1 << (n_big - 5)) * 3 * 2 * 32 * 32
+    mfma_flops = nb * (1 << (n_big - 5)) * 4 * 2 * 32 * 32
     t0 = time.perf_counter()
     e = sv.expval(ham)
     t_h = time.perf_counter() - t0

@@ -1,6 +1,6 @@
 """GPU parity: every gate, generator and dense matrix through the C-ABI
 against the oracle (Lightning-Qubit semantics) on the same seeded states.
-Tolerance: 1e-12 relative for complex128; 1e-5 for complex64."""
+Tolerance: 1e-12 relative for complex128; 2e-5 for complex64."""
 import itertools
 
 import numpy as np
