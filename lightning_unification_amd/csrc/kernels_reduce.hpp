@@ -22,8 +22,8 @@ __device__ __forceinline__ double wave_sum(double x) {
 }
 
 // Sum of two doubles over a 256-thread block; result valid in thread 0.
-__device__ __forceinline__ double2 block_sum2(double2 v) {
-    __shared__ double sx[kBlock / 64], sy[kBlock / 64];
+template <int NT = kBlock> __device__ __forceinline__ double2 block_sum2(double2 v) {
+    __shared__ double sx[NT / 64], sy[NT / 64];
     v.x = wave_sum(v.x);
     v.y = wave_sum(v.y);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -34,7 +34,7 @@ __device__ __forceinline__ double2 block_sum2(double2 v) {
     __syncthreads();
     double2 r{0, 0};
     if (threadIdx.x == 0) {
-        for (int w = 0; w < kBlock / 64; w++) {
+        for (int w = 0; w < NT / 64; w++) {
             r.x += sx[w];
             r.y += sy[w];
         }
@@ -308,11 +308,14 @@ __global__ __launch_bounds__(kBlock) void k_sparse_moments(const v2<T> *__restri
 // bits fixed).  Tiles are staged in LDS once; each group then walks its pairs
 // there.  A term's Z/Y parity splits into the tile-local bits (zyl, in the
 // compressed tile index) and the bits outside S (zyo), constant per tile.
+// High bits per tile: 10 (tiles of 2^13 amplitudes, 1024-thread workgroups)
+// where a workgroup's LDS holds such a tile, else 8 (2^11, 256 threads).
 constexpr int kPauliTileHigh = 8;
+constexpr int kPauliTileHighMax = 10;
 struct PauliTileGeom {
-    int nhigh;                    // high bits in S
-    int high[kPauliTileHigh];     // their positions, ascending (all >= 3)
-    int nlow;                     // low bits in S: min(3, n)
+    int nhigh;                       // high bits in S
+    int high[kPauliTileHighMax];     // their positions, ascending (all >= 3)
+    int nlow;                        // low bits in S: min(3, n)
 };
 // The terms of a group are sorted into buckets by the part of zyl that the
 // thread's own elements differ in (pauli_bucket); bucket k is the term range
@@ -335,7 +338,7 @@ struct PauliTileTerm {
     int pad;
 };
 
-constexpr int kPauliPerThread = 8; // amplitudes a thread loads per tile (2^11 / 256)
+constexpr int kPauliPerThread = 8; // amplitudes a thread loads per tile (2^(3 + HIGH) / 2^HIGH)
 constexpr int kPauliPairs = kPauliPerThread / 2;
 
 // Tile-local element e with a zero inserted at bit lb (the pair's low end).
@@ -343,20 +346,22 @@ __host__ __device__ inline uint32_t pauli_insert0(uint32_t e, int lb) {
     const uint32_t m = (1U << lb) - 1;
     return ((e & ~m) << 1) | (e & m);
 }
-// Thread t of a workgroup owns the elements t + 256 j (diagonal group, j < 8)
-// or the pairs whose low ends are pauli_insert0(t + 256 j, lb) (j < 4).  They
-// differ from the first one in bits fixed by j alone, so the sign of a term at
-// element j is its sign at the first element times (-1)^popcount(j & k), k
-// made of the term's zyl bits at those positions: the bucket of the term.
-__host__ __device__ inline int pauli_bucket(uint32_t xl, uint32_t zyl) {
+// Thread t of a workgroup of NT = 2^HIGH threads owns the elements t + NT j
+// (diagonal group, j < 8) or the pairs whose low ends are
+// pauli_insert0(t + NT j, lb) (j < 4).  They differ from the first one in bits
+// fixed by j alone, so the sign of a term at element j is its sign at the
+// first element times (-1)^popcount(j & k), k made of the term's zyl bits at
+// those positions: the bucket of the term.
+__host__ __device__ inline int pauli_bucket(uint32_t xl, uint32_t zyl, int high) {
+    const uint32_t nt = 1U << high;
     if (xl == 0) {
-        return static_cast<int>((zyl >> 8) & 7U);
+        return static_cast<int>((zyl >> high) & 7U);
     }
     int lb = 0;
     while (!((xl >> lb) & 1U)) {
         lb++;
     }
-    const uint32_t d1 = pauli_insert0(256, lb), d2 = pauli_insert0(512, lb);
+    const uint32_t d1 = pauli_insert0(nt, lb), d2 = pauli_insert0(2 * nt, lb);
     return ((zyl & d1) ? 1 : 0) | ((zyl & d2) ? 2 : 0);
 }
 
@@ -377,9 +382,10 @@ __device__ __forceinline__ double pauli_flip(double v, uint32_t par) {
 // outside S included): one AND, one popcount, a sign flip and two FMAs per
 // term and thread, not per pair.  Terms and groups are read by uniform index
 // (scalar loads).
-// FULL: every tile holds 2^11 amplitudes (no element or pair is out of range)
-template <class T, bool FULL>
-__global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
+// HIGH: high bits of a full tile, 2^HIGH threads per workgroup.  FULL: every
+// tile holds 2^(3 + HIGH) amplitudes (no element or pair is out of range).
+template <class T, int HIGH, bool FULL>
+__global__ __launch_bounds__(1 << HIGH) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const PauliTileGeom geo, int ngroups,
                                                        const PauliTileGroup *__restrict__ groups,
                                                        const PauliTileTerm *__restrict__ terms,
@@ -387,14 +393,15 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
     const int nS = geo.nlow + geo.nhigh;
-    const uint32_t tsz = FULL ? (1U << (3 + kPauliTileHigh)) : (1U << nS);
+    constexpr int NT = 1 << HIGH;
+    const uint32_t tsz = FULL ? (1U << (3 + HIGH)) : (1U << nS);
     const uint32_t lowmask = (1U << geo.nlow) - 1;
     const uint32_t tid = threadIdx.x;
     // the tile-local part of each loaded element's state index
     uint64_t off[kPauliPerThread];
 #pragma unroll
     for (int k = 0; k < kPauliPerThread; k++) {
-        const uint32_t e = tid + k * kBlock;
+        const uint32_t e = tid + k * NT;
         uint64_t g = e & lowmask;
         for (int b = 0; b < geo.nhigh; b++) {
             g |= static_cast<uint64_t>((e >> (geo.nlow + b)) & 1U) << geo.high[b];
@@ -416,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
     if (q < ntiles) {
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (FULL || tid + k * kBlock < tsz) {
+            if (FULL || tid + k * NT < tsz) {
                 nxt[k] = ld_nt<T>(&sv[base | off[k]]);
             }
         }
@@ -427,8 +434,8 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
         __syncthreads();           // the previous tile is no longer read
 #pragma unroll
         for (int k = 0; k < kPauliPerThread; k++) {
-            if (FULL || tid + k * kBlock < tsz) {
-                tile[tid + k * kBlock] = nxt[k];
+            if (FULL || tid + k * NT < tsz) {
+                tile[tid + k * NT] = nxt[k];
             }
         }
         __syncthreads();
@@ -437,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             base = tile_base(qn);
 #pragma unroll
             for (int k = 0; k < kPauliPerThread; k++) {
-                if (FULL || tid + k * kBlock < tsz) {
+                if (FULL || tid + k * NT < tsz) {
                     nxt[k] = ld_nt<T>(&sv[base | off[k]]);
                 }
             }
@@ -456,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
                 double w[8];
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    const uint32_t e = tid + j * kBlock;
+                    const uint32_t e = tid + j * NT;
                     w[j] = 0;
                     if (FULL || e < tsz) {
                         const v2<T> a = tile[e];
@@ -498,7 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             const uint32_t e0 = pauli_insert0(tid, lb);
             // the thread's pairs: low ends e0 ^ d_j, partners e0 ^ d_j ^ xl,
             // d_j = pauli_insert0(256 j, lb) uniform (its bits are not e0's)
-            const uint32_t d1 = pauli_insert0(kBlock, lb), d2 = pauli_insert0(2 * kBlock, lb);
+            const uint32_t d1 = pauli_insert0(NT, lb), d2 = pauli_insert0(2 * NT, lb);
             const char *tb8 = reinterpret_cast<const char *>(tile);
             const uint32_t e0b = e0 * static_cast<uint32_t>(sizeof(v2<T>));
             double zr[kPauliPairs], zi[kPauliPairs];
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             for (int j = 0; j < kPauliPairs; j++) {
                 const uint32_t dj = ((j & 1) ? d1 : 0U) | ((j & 2) ? d2 : 0U);
                 zr[j] = zi[j] = 0;
-                if (FULL || tid + j * kBlock < npairs) {
+                if (FULL || tid + j * NT < npairs) {
                     const v2<T> a = *reinterpret_cast<const v2<T> *>(tb8 + (e0b ^ (dj * sizeof(v2<T>))));
                     const v2<T> c = *reinterpret_cast<const v2<T> *>(tb8 + (e0b ^ ((dj ^ xl) * sizeof(v2<T>))));
                     // z = conj(a) * c
@@ -543,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_pauli_tile(const v2<T> *__restrict__
             }
         }
     }
-    const double2 r = block_sum2(double2{acc, 0});
+    const double2 r = block_sum2<NT>(double2{acc, 0});
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = r;
     }

@@ -561,6 +561,16 @@ template <class T> struct Engine {
     }
 
     // sum_t c_t <psi|P_t|psi> over Pauli words, grouped by x-mask
+    // High bits per Pauli tile: 10 where a workgroup's LDS holds a tile of
+    // 2^13 amplitudes (fewer passes), else 8.
+    int pauli_high_bits() {
+        int dev = 0, lds_max = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t need = (size_t{1} << (3 + kPauliTileHighMax)) * sizeof(V) + 1024; // + block_sum2's arrays
+        return static_cast<size_t>(lds_max) >= need ? kPauliTileHighMax : kPauliTileHigh;
+    }
+
     // sum_t c_t <psi|P_t|psi> over Pauli words.  Distinct x-masks are packed
     // greedily into passes whose high bits fit one LDS tile (k_pauli_tile);
     // each pass reads the state once.  Masks wider than a tile get a pass of
@@ -576,6 +586,7 @@ template <class T> struct Engine {
         }
         const int nlow = std::min(3, n());
         const uint64_t lowm = (uint64_t{1} << nlow) - 1;
+        const int hmax = pauli_high_bits();
         // work items: an x-mask with all of its terms
         struct Item {
             uint64_t x;
@@ -584,7 +595,7 @@ template <class T> struct Engine {
         std::vector<Item> items;
         std::vector<uint64_t> wide;
         for (auto &kv : groups) {
-            if (kv.first && __builtin_popcountll(kv.first & ~lowm) > kPauliTileHigh) {
+            if (kv.first && __builtin_popcountll(kv.first & ~lowm) > hmax) {
                 wide.push_back(kv.first);
                 continue;
             }
@@ -608,7 +619,7 @@ template <class T> struct Engine {
             std::vector<const Item *> left;
             for (const Item *it : rest) {
                 const uint64_t xh = it->x & ~lowm;
-                if (__builtin_popcountll(ps.high | xh) <= kPauliTileHigh) {
+                if (__builtin_popcountll(ps.high | xh) <= hmax) {
                     ps.high |= xh;
                     ps.its.push_back(it);
                 } else {
@@ -622,7 +633,7 @@ template <class T> struct Engine {
         // with the lowest free positions (a pass whose masks need one high bit
         // would otherwise walk 2^(n-4) tiles of 16 amplitudes)
         for (auto &ps : passes) {
-            for (int b = nlow; b < n() && __builtin_popcountll(ps.high) < kPauliTileHigh; b++) {
+            for (int b = nlow; b < n() && __builtin_popcountll(ps.high) < hmax; b++) {
                 ps.high |= uint64_t{1} << b;
             }
         }
@@ -673,7 +684,7 @@ template <class T> struct Engine {
                             continue; // contributes exactly zero
                         }
                     }
-                    bucket[pauli_bucket(gr.xl, zyl)].push_back(PauliTileTerm{pt.zy & ~smask, cr, ci, zyl, 0});
+                    bucket[pauli_bucket(gr.xl, zyl, hmax)].push_back(PauliTileTerm{pt.zy & ~smask, cr, ci, zyl, 0});
                 }
                 for (int k = 0; k < nb; k++) {
                     gr.tb[k] = static_cast<int>(ttab.size());
@@ -722,15 +733,27 @@ template <class T> struct Engine {
             const PauliTileGeom &g = geos[pi];
             const int nS = g.nlow + g.nhigh;
             const uint64_t ntiles = len() >> nS;
-            const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(ntiles, kReduceBlocks));
             const size_t lds = (size_t{1} << nS) * sizeof(V);
             const int ng = static_cast<int>(gspan[pi].second);
-            if (nS == 3 + kPauliTileHigh) { // full tiles (every state of >= 11 qubits)
-                hipLaunchKernelGGL((k_pauli_tile<T, true>), dim3(grid), dim3(kBlock), lds, h->stream, sv, ntiles, g,
-                                   ng, dgroups + gspan[pi].first, dtterms, part + total);
+            const bool full = nS == 3 + hmax; // every state of >= 3 + hmax qubits
+            auto launch = [&](auto kernel, int threads) {
+                // persistent workgroups: as many as the device holds at once
+                int per_cu = 0;
+                HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds));
+                const uint64_t resident = static_cast<uint64_t>(device_cus()) * std::max(per_cu, 1);
+                const unsigned grid = static_cast<unsigned>(
+                    std::min<uint64_t>({ntiles, resident, static_cast<uint64_t>(kReduceBlocks)}));
+                hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), lds, h->stream, sv, ntiles, g, ng,
+                                   dgroups + gspan[pi].first, dtterms, part + total);
+                return grid;
+            };
+            unsigned grid = 0;
+            if (hmax == kPauliTileHighMax) {
+                grid = full ? launch(k_pauli_tile<T, kPauliTileHighMax, true>, 1 << kPauliTileHighMax)
+                            : launch(k_pauli_tile<T, kPauliTileHighMax, false>, 1 << kPauliTileHighMax);
             } else {
-                hipLaunchKernelGGL((k_pauli_tile<T, false>), dim3(grid), dim3(kBlock), lds, h->stream, sv, ntiles, g,
-                                   ng, dgroups + gspan[pi].first, dtterms, part + total);
+                grid = full ? launch(k_pauli_tile<T, kPauliTileHigh, true>, 1 << kPauliTileHigh)
+                            : launch(k_pauli_tile<T, kPauliTileHigh, false>, 1 << kPauliTileHigh);
             }
             HIP_OK(hipGetLastError());
             total += grid;
