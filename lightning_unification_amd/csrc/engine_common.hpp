@@ -249,7 +249,7 @@ struct ScratchScope {
 
 // Device buffers kept by a handle between calls, one per purpose.
 enum Buf { kBufUniforms, kBufIndices, kBufBits, kBufCdfGroups, kBufCdfPrefix, kBufCdfTiles, kBufChainStart,
-           kBufScalar, kBufCount };
+           kBufScalar, kBufVjp, kBufCount };
 
 // The handle's buffer for `purpose`, at least `bytes` long (grown when
 // needed, after the handle's queued work, which may still use the old one).

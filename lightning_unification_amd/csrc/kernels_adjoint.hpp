@@ -222,4 +222,98 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_final(const double *__restri
     }
 }
 
+// Fused step of the vector-Jacobian product's backward sweep
+// (VectorJacobianProduct.hpp:77-155): per amplitude group of the op, load
+// lambda's and mu's groups; if the op is trainable, accumulate the complex
+// <mu|G lambda> (= <G mu|lambda>, G Hermitian and zero outside the op's
+// geometry); store U^dag lambda and U^dag mu.  One launch per op, each state
+// read and written once; partial[blockIdx.x] = the block's sum.
+template <class T, int M, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_vjp_step(v2<T> *__restrict__ lam, v2<T> *__restrict__ mu,
+                                                     uint64_t nwork, const AdjOp<T, M> op,
+                                                     double2 *__restrict__ partial) {
+    double2 acc{0, 0};
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; g < nwork; g += stride) {
+        const uint64_t base = expand(g, op.geo);
+        v2<T> x[M], h[M], y[M];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            x[j] = lam[base | op.off[j]];
+            h[j] = mu[base | op.off[j]];
+        }
+        if (TRAIN) {
+            v2<T> gx[M];
+            matvec<T, M>(op.g, 0, x, gx);
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                // conj(h) * gx
+                acc.x += double(h[j].x) * double(gx[j].x) + double(h[j].y) * double(gx[j].y);
+                acc.y += double(h[j].x) * double(gx[j].y) - double(h[j].y) * double(gx[j].x);
+            }
+        }
+        matvec<T, M>(op.ud, op.ud_diag, x, y);
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            lam[base | op.off[j]] = y[j];
+        }
+        matvec<T, M>(op.ud, op.ud_diag, h, y);
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            mu[base | op.off[j]] = y[j];
+        }
+    }
+    if (TRAIN) {
+        const double2 r = block_sum2(acc);
+        if (threadIdx.x == 0) {
+            partial[blockIdx.x] = r;
+        }
+    }
+}
+
+// The same for parity-phase ops (MultiRZ): U^dag and G are phases selected
+// by the parity of i & mask.
+template <class T, bool TRAIN>
+__global__ __launch_bounds__(kBlock) void k_vjp_parity(v2<T> *__restrict__ lam, v2<T> *__restrict__ mu,
+                                                       uint64_t len, uint64_t mask, dcx<T> u0, dcx<T> u1,
+                                                       dcx<T> g0, dcx<T> g1, double2 *__restrict__ partial) {
+    double2 acc{0, 0};
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len; i += stride) {
+        const bool odd = __popcll(i & mask) & 1;
+        const v2<T> x = lam[i], h = mu[i];
+        if (TRAIN) {
+            const v2<T> gx = cmul(odd ? g1 : g0, x);
+            acc.x += double(h.x) * double(gx.x) + double(h.y) * double(gx.y);
+            acc.y += double(h.x) * double(gx.y) - double(h.y) * double(gx.x);
+        }
+        const dcx<T> u = odd ? u1 : u0;
+        lam[i] = cmul(u, x);
+        mu[i] = cmul(u, h);
+    }
+    if (TRAIN) {
+        const double2 r = block_sum2(acc);
+        if (threadIdx.x == 0) {
+            partial[blockIdx.x] = r;
+        }
+    }
+}
+
+// out[rows[b]] = sum over i < nblocks of partial[rows[b] * nblocks + i], in
+// a fixed order: one block per row
+__global__ __launch_bounds__(kBlock) void k_vjp_final(const double2 *__restrict__ partial, int nblocks,
+                                                      const int *__restrict__ rows, double2 *__restrict__ out) {
+    const int t = rows[blockIdx.x];
+    double2 s{0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+        const double2 p = partial[static_cast<size_t>(t) * nblocks + b];
+        s.x += p.x;
+        s.y += p.y;
+    }
+    const double2 r = block_sum2(s);
+    if (threadIdx.x == 0) {
+        out[t] = r;
+    }
+}
+
 } // namespace plamd

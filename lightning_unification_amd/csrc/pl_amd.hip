@@ -1431,7 +1431,9 @@ using namespace plamd;
 // lambda = psi, mu = dy; for operations in reverse (state preparations
 // skipped, stopping once every trainable parameter is done): for a trainable
 // one, w_t = i s <G mu | lambda> with s the generator scale times -1 for an
-// inverted operation; then lambda <- U^dag lambda and mu <- U^dag mu.
+// inverted operation; then lambda <- U^dag lambda and mu <- U^dag mu.  Ops of
+// at most 4 members (and parity phases) whose generator shares their
+// geometry take one fused launch each (k_vjp_step); the rest go op by op.
 template <class T>
 void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const int *wires,
                const uint8_t *inverse, const int *nparams, const double *params, const double *dy,
@@ -1478,6 +1480,11 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
     double2 *dots = part + kReduceBlocks;
     std::vector<double> scale(ntp, 0.0);
     std::vector<char> reached(ntp, 0);
+    // fused steps (k_vjp_step / k_vjp_parity): one launch per op; a trainable
+    // op's block partials go to its row of vpart, all rows reduced at the end
+    const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 8), kReduceBlocks);
+    auto *vpart = static_cast<double2 *>(buf(h, kBufVjp, size_t(ntp) * grid * sizeof(double2)));
+    std::vector<int> fused_rows;
     for (int i = nops - 1; i >= 0; i--) {
         if (nparams[i] > 1) {
             throw InvalidArg("The operation is not supported using the adjoint differentiation method");
@@ -1489,23 +1496,97 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
             break;
         }
         const int *w = wires + woff[i];
+        const auto pt = E.params_t(params + poff[i], nparams[i]);
+        const GroupOp<T> ud = lower_gate<T>(E.n(), ids[i], w, nwires[i], inverse[i] == 0, pt.data(),
+                                            nparams[i]);
+        bool train = false;
+        int row = -1;
+        GroupOp<T> g;
         if (nparams[i] > 0) {
             if (cur_param == tp[tp_pos]) {
-                E.copy(mud, mu);
+                train = true;
+                row = tp_pos;
                 T s = 0;
-                E.apply(lower_generator<T>(E.n(), ids[i], w, nwires[i], false, &s), mud);
+                g = lower_generator<T>(E.n(), ids[i], w, nwires[i], true, &s);
                 scale[tp_pos] = static_cast<double>(s) * (inverse[i] ? -1.0 : 1.0);
                 reached[tp_pos] = 1;
-                E.dot_to(mud, lam, part, dots + tp_pos);
                 tp_pos--;
             }
             cur_param--;
         }
-        const auto pt = E.params_t(params + poff[i], nparams[i]);
-        const GroupOp<T> ud = lower_gate<T>(E.n(), ids[i], w, nwires[i], inverse[i] == 0, pt.data(),
-                                            nparams[i]);
-        E.apply(ud, lam);
-        E.apply(ud, mu);
+        const bool fusable = (ud.kind == GroupOp<T>::Noop && !train) ||
+                             (ud.kind == GroupOp<T>::Parity && (!train || g.same_geometry(ud))) ||
+                             (ud.kind == GroupOp<T>::Group && ud.M <= 4 && (!train || g.same_geometry(ud)));
+        if (!fusable) {
+            if (train) {
+                E.copy(mud, mu);
+                T s = 0;
+                E.apply(lower_generator<T>(E.n(), ids[i], w, nwires[i], false, &s), mud);
+                E.dot_to(mud, lam, part, dots + row);
+            }
+            E.apply(ud, lam);
+            E.apply(ud, mu);
+            continue;
+        }
+        if (ud.kind == GroupOp<T>::Noop) {
+            continue;
+        }
+        double2 *pp = train ? vpart + size_t(row) * grid : nullptr;
+        if (ud.kind == GroupOp<T>::Parity) {
+            const dcx<T> u0{ud.parity_phase[0].real(), ud.parity_phase[0].imag()};
+            const dcx<T> u1{ud.parity_phase[1].real(), ud.parity_phase[1].imag()};
+            dcx<T> g0{1, 0}, g1{1, 0};
+            if (train) {
+                g0 = {g.parity_phase[0].real(), g.parity_phase[0].imag()};
+                g1 = {g.parity_phase[1].real(), g.parity_phase[1].imag()};
+                hipLaunchKernelGGL((k_vjp_parity<T, true>), dim3(grid), dim3(kBlock), 0, h->stream, lam, mu,
+                                   E.len(), ud.parity_mask, u0, u1, g0, g1, pp);
+            } else {
+                hipLaunchKernelGGL((k_vjp_parity<T, false>), dim3(grid), dim3(kBlock), 0, h->stream, lam, mu,
+                                   E.len(), ud.parity_mask, u0, u1, g0, g1, pp);
+            }
+        } else {
+            auto go = [&](auto mtag) {
+                constexpr int M = decltype(mtag)::value;
+                AdjOp<T, M> a{};
+                a.geo.fixed = ud.fixed;
+                a.geo.nb = ud.nb;
+                for (int q = 0; q < ud.nb; q++) {
+                    a.geo.pos[q] = ud.pos[q];
+                }
+                for (int j = 0; j < M; j++) {
+                    a.off[j] = ud.off[j];
+                }
+                for (int j = 0; j < M * M; j++) {
+                    a.ud[j] = {ud.mat[j].real(), ud.mat[j].imag()};
+                    a.g[j] = train ? dcx<T>{g.mat[j].real(), g.mat[j].imag()} : dcx<T>{0, 0};
+                }
+                a.ud_diag = ud.diag ? 1 : 0;
+                const uint64_t nwork = E.len() >> ud.nb;
+                if (train) {
+                    hipLaunchKernelGGL((k_vjp_step<T, M, true>), dim3(grid), dim3(kBlock), 0, h->stream, lam, mu,
+                                       nwork, a, pp);
+                } else {
+                    hipLaunchKernelGGL((k_vjp_step<T, M, false>), dim3(grid), dim3(kBlock), 0, h->stream, lam, mu,
+                                       nwork, a, pp);
+                }
+            };
+            switch (ud.M) {
+            case 1: go(std::integral_constant<int, 1>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            default: go(std::integral_constant<int, 4>{}); break;
+            }
+        }
+        HIP_OK(hipGetLastError());
+        if (train) {
+            fused_rows.push_back(row);
+        }
+    }
+    if (!fused_rows.empty()) {
+        const auto *drows = static_cast<const int *>(stage(h, fused_rows.data(), fused_rows.size() * sizeof(int)));
+        hipLaunchKernelGGL(k_vjp_final, dim3(static_cast<unsigned>(fused_rows.size())), dim3(kBlock), 0, h->stream,
+                           vpart, static_cast<int>(grid), drows, dots);
+        HIP_OK(hipGetLastError());
     }
     std::vector<double2> z(ntp);
     HIP_OK(hipMemcpyAsync(z.data(), dots, ntp * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
