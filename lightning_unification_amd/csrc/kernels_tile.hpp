@@ -192,7 +192,6 @@ template <class T> __device__ __forceinline__ dcx<T> tp_ld(const dcx<T> &a) {
 enum TrCode : int {
     kTrDiag = 1, kTrParity = 2, kTrP1 = 3, kTrP2 = 4, kTrP4 = 5, kTrLds = 6, kTrLane = 7, kTrMix1 = 8, kTrMix2 = 9
 };
-constexpr int kTrMaxKB = kTpMaxBits - 6;
 
 template <class T> struct TrInst {
     int code;
