@@ -504,7 +504,7 @@ __global__ __launch_bounds__(1 << HIGH) void k_pauli_tile(const v2<T> *__restric
             const uint32_t npairs = tsz >> 1;
             const uint32_t e0 = pauli_insert0(tid, lb);
             // the thread's pairs: low ends e0 ^ d_j, partners e0 ^ d_j ^ xl,
-            // d_j = pauli_insert0(256 j, lb) uniform (its bits are not e0's)
+            // d_j = pauli_insert0(NT j, lb) uniform (its bits are not e0's)
             const uint32_t d1 = pauli_insert0(NT, lb), d2 = pauli_insert0(2 * NT, lb);
             const char *tb8 = reinterpret_cast<const char *>(tile);
             const uint32_t e0b = e0 * static_cast<uint32_t>(sizeof(v2<T>));
