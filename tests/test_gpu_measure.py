@@ -118,22 +118,33 @@ def test_sampling_bit_exact(oracle, dtype, n):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-def test_pauli_sum_tiled_passes(oracle, dtype):
-    """Many Pauli words: x-masks packed into tiled passes (high bits of up to
-    eight per tile), words too wide for a tile, and Z-only words, against the
-    oracle's term-by-term evaluation."""
+@pytest.mark.parametrize("n", [11, 12, 14])
+def test_pauli_sum_tiled_passes(oracle, dtype, n):
+    """Many Pauli words: x-masks packed into tiled passes (partial tiles at
+    11 and 12 qubits, full ones at 14), words too wide for a tile, Z-only
+    words, and many words sharing one x-mask with different Z/Y bits (so the
+    per-thread Walsh-Hadamard transform runs in full as well as for one or
+    two buckets), against the oracle's term-by-term evaluation."""
     from lightning_unification_amd import capi
     from lightning_unification_amd.circuits import Obs, pauli_word
-    rs = np.random.default_rng(77)
-    n = 14
+    rs = np.random.default_rng(77 + n)
     words, coeffs = [], []
     for k in range(60):
-        wt = int(rs.integers(1, 13)) if k % 5 else int(rs.integers(1, 4))
+        wt = int(rs.integers(1, min(n, 13))) if k % 5 else int(rs.integers(1, 4))
         wires = rs.choice(n, wt, replace=False).tolist()
         letters = "".join("XYZ"[int(rs.integers(3))] for _ in range(wt))
         if k % 7 == 0:
             letters = "Z" * wt
         words.append(pauli_word(letters, wires))
+        coeffs.append(float(rs.normal()))
+    # one x-mask (X on wires 0 and n-1) with Z factors on every other wire
+    # subset drawn at random, and Z-only words on random subsets
+    for k in range(24):
+        zs = [w for w in range(1, n - 1) if rs.uniform() < 0.5]
+        words.append(pauli_word("X" + "Z" * len(zs) + "X", [0] + zs + [n - 1]))
+        coeffs.append(float(rs.normal()))
+        zw = [w for w in range(n) if rs.uniform() < 0.4] or [0]
+        words.append(pauli_word("Z" * len(zw), zw))
         coeffs.append(float(rs.normal()))
     ham = Obs.hamiltonian(coeffs, words)
     s = random_state(n, 5, dtype)
