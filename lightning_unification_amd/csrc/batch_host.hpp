@@ -75,7 +75,7 @@ struct Gf2Map {
 template <class T> struct BatchPlan final : BatchPlanBase {
     using V = v2<T>;
     struct LaunchSpec {
-        enum Kind { Init, Rot, Ring, Obs, RingAdj, RotAdj } kind;
+        enum Kind { Init, Rot, Ring, Obs, RotAdj } kind;
         int layer = 0, stage = 0;
         BLaunch L{};
         uint64_t blocks = 0;
@@ -296,15 +296,16 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                     [&](int j) { return depth[j] > l && stages(nq[j], W) > s; },
                     [&](int j) { return uint64_t{1} << (nq[j] - stage_T(nq[j], W, s)); });
             }
+            // a layer's entangler is fused into the next layer's first
+            // rotation stage; only a circuit's last one runs on its own
             if (entangler) {
-                add(LaunchSpec::Ring, l, 0, [&](int j) { return depth[j] > l; }, elem_blocks);
+                add(LaunchSpec::Ring, l, 0, [&](int j) { return depth[j] == l + 1; }, elem_blocks);
             }
         }
         add(LaunchSpec::Obs, 0, 0, [&](int) { return true; }, elem_blocks);
+        // the entangler of each layer is undone inside its first adjoint
+        // rotation stage (k_batch_rot_adj gathers lambda and H)
         for (int l = dmax - 1; l >= 0; l--) {
-            if (entangler) {
-                add(LaunchSpec::RingAdj, l, 0, [&](int j) { return depth[j] > l; }, elem_blocks);
-            }
             for (int s = smax - 1; s >= 0; s--) {
                 add(LaunchSpec::RotAdj, l, s,
                     [&](int j) { return depth[j] > l && stages(nq[j], W) > s; },
@@ -346,10 +347,13 @@ template <class T> struct BatchPlan final : BatchPlanBase {
             case LaunchSpec::Init:
                 hipLaunchKernelGGL((k_batch_init<T>), grid, dim3(kBlock), 0, stream, arena, dcircs, ls.L);
                 break;
-            case LaunchSpec::Rot:
+            case LaunchSpec::Rot: {
+                // stage 0 of layer l >= 1 gathers through layer l-1's entangler
+                const int gsrc = (entangler && ls.layer > 0 && ls.stage == 0) ? nxt : -1;
                 hipLaunchKernelGGL((k_batch_rot<T>), grid, dim3(kBlock), tile_bytes, stream, arena,
-                                   dcircs, ls.L, W, ls.stage, ls.layer, cur, npat, pat, dcs);
+                                   dcircs, ls.L, W, ls.stage, ls.layer, cur, gsrc, npat, pat, dcs);
                 break;
+            }
             case LaunchSpec::Ring:
                 hipLaunchKernelGGL((k_batch_ring<T>), grid, dim3(kBlock), 0, stream, arena, dcircs,
                                    ls.L, 0, cur, nxt, -1, -1);
@@ -359,15 +363,12 @@ template <class T> struct BatchPlan final : BatchPlanBase {
                                    ls.L, entangler ? 1 : 0, dterms, dxm, denergy,
                                    denergy + nc);
                 break;
-            case LaunchSpec::RingAdj:
-                // undo the entangler of layer l: buffers (l+1)%2 -> l%2 for lambda and H
-                hipLaunchKernelGGL((k_batch_ring<T>), grid, dim3(kBlock), 0, stream, arena, dcircs,
-                                   ls.L, 1, nxt, cur, 2 + nxt, 2 + cur);
-                break;
             case LaunchSpec::RotAdj:
+                // lambda and H of layer l+1's side (buffers nxt) are gathered
+                // through the inverse entangler at each circuit's first stage
                 hipLaunchKernelGGL((k_batch_rot_adj<T>), grid, dim3(kBlock), 2 * tile_bytes, stream,
-                                   arena, dcircs, ls.L, W, ls.stage, ls.layer, cur, 2 + cur, npat, pat,
-                                   dcs, dgrads);
+                                   arena, dcircs, ls.L, W, ls.stage, ls.layer, cur, 2 + cur,
+                                   entangler ? nxt : -1, entangler ? 2 + nxt : -1, npat, pat, dcs, dgrads);
                 break;
             }
             const hipError_t err = hipGetLastError();

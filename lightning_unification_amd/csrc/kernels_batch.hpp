@@ -197,13 +197,25 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t q, const int *bits, int T
     return g;
 }
 
+// source index of y under an entangler map: bit b = parity(y & rows[b])
+__device__ __forceinline__ uint64_t gather_src(uint64_t y, const uint32_t *rows, int n) {
+    uint64_t x = 0;
+    for (int b = 0; b < n; b++) {
+        x |= static_cast<uint64_t>(__popc(static_cast<uint32_t>(y) & rows[b]) & 1) << b;
+    }
+    return x;
+}
+
 // Forward rotation stage: for every target wire of the tile, the product of
-// its pattern rotations (one 2x2 per wire) is applied in LDS.
+// its pattern rotations (one 2x2 per wire) is applied in LDS.  gsrc >= 0: the
+// previous layer's entangler is fused in -- the tile is gathered from buffer
+// gsrc through the entangler's map and written to buffer buf (so the
+// entangler costs no pass of its own).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_rot(v2<T> *__restrict__ arena,
                                                       const BCirc *__restrict__ circs,
                                                       const BLaunch L, int W, int stage, int layer,
-                                                      int buf, int npat, int4 pat,
+                                                      int buf, int gsrc, int npat, int4 pat,
                                                       const double2 *__restrict__ cs) {
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
@@ -216,8 +228,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot(v2<T> *__restrict__ arena,
     v2<T> *sv = arena + c.off + buf * len;
     const uint64_t base = tile_base(q, bits, Tn);
     const uint32_t tsz = 1U << Tn;
-    for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
-        tile[e] = sv[tile_index(base, e, bits, Tn)];
+    if (gsrc >= 0) {
+        const v2<T> *src = arena + c.off + gsrc * len;
+        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+            tile[e] = src[gather_src(tile_index(base, e, bits, Tn), c.fwd_rows, c.n)];
+        }
+    } else {
+        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+            tile[e] = sv[tile_index(base, e, bits, Tn)];
+        }
     }
     __syncthreads();
     const int axes[4] = {pat.x, pat.y, pat.z, pat.w};
@@ -245,14 +264,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot(v2<T> *__restrict__ arena,
     for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
         sv[tile_index(base, e, bits, Tn)] = tile[e];
     }
-}
-
-__device__ __forceinline__ uint64_t gather_src(uint64_t y, const uint32_t *rows, int n) {
-    uint64_t x = 0;
-    for (int b = 0; b < n; b++) {
-        x |= static_cast<uint64_t>(__popc(static_cast<uint32_t>(y) & rows[b]) & 1) << b;
-    }
-    return x;
 }
 
 // Entangler as a gather: dst[y] = src[src_index(y)] for the buffers given.
@@ -362,12 +373,16 @@ __global__ __launch_bounds__(kBlock) void k_batch_obs(v2<T> *__restrict__ arena,
 // Im<H|G lambda>, AdjointJacobianLQubit.hpp:70-75).  Each wave reduces its
 // share with shuffles into a per-wave LDS slot; after the last wire one pass
 // adds the four waves' slots and issues one FP64 atomic per angle.
+// glb >= 0: the layer's entangler is undone in the same pass -- for a
+// circuit whose first adjoint stage of the layer this is (stage ==
+// nstage - 1), lambda and H are gathered from buffers glb and ghb through the
+// inverse map and written to lb and hb.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ arena,
                                                           const BCirc *__restrict__ circs,
                                                           const BLaunch L, int W, int stage,
-                                                          int layer, int lb, int hb, int npat,
-                                                          int4 pat, const double2 *__restrict__ cs,
+                                                          int layer, int lb, int hb, int glb, int ghb,
+                                                          int npat, int4 pat, const double2 *__restrict__ cs,
                                                           double *__restrict__ grads) {
     extern __shared__ unsigned char smem_raw[];
     __shared__ double wpart[kBlock / 64][kBatchMaxTile][kBatchMaxPattern];
@@ -383,10 +398,19 @@ __global__ __launch_bounds__(kBlock) void k_batch_rot_adj(v2<T> *__restrict__ ar
     v2<T> *lam = arena + c.off + lb * len;
     v2<T> *hh = arena + c.off + hb * len;
     const uint64_t base = tile_base(q, bits, Tn);
-    for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
-        const uint64_t g = tile_index(base, e, bits, Tn);
-        tl[e] = lam[g];
-        th_[e] = hh[g];
+    if (glb >= 0 && stage == c.nstage - 1) { // uniform per workgroup
+        const v2<T> *ls = arena + c.off + glb * len, *hs = arena + c.off + ghb * len;
+        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+            const uint64_t x = gather_src(tile_index(base, e, bits, Tn), c.adj_rows, c.n);
+            tl[e] = ls[x];
+            th_[e] = hs[x];
+        }
+    } else {
+        for (uint32_t e = threadIdx.x; e < tsz; e += kBlock) {
+            const uint64_t g = tile_index(base, e, bits, Tn);
+            tl[e] = lam[g];
+            th_[e] = hh[g];
+        }
     }
     __syncthreads();
     const int axes[4] = {pat.x, pat.y, pat.z, pat.w};
