@@ -253,8 +253,8 @@ int pl_fuse_tape(int num_qubits, int nops, const int *ids, const int *nwires,
  * tile_bits in [9, 12] ([9, 11] for complex128: a 2^12-amplitude tile does
  * not fit a wavefront's registers) and <= the qubit count; low_bits in
  * [0, 6].  tile_bits = 0 and low_bits = -1 pick the
- * defaults of the precision (2^10 amplitudes and 3 low bits for complex128,
- * 2^11 and 4 for complex64, clamped to the state); with tile_bits = 0 a state
+ * defaults of the precision (2^11 amplitudes and 3 low bits for complex128,
+ * 2^12 and 4 for complex64, clamped to the state); with tile_bits = 0 a state
  * of fewer than 9 qubits runs gate by gate.  flags: PL_FUSE_RENORM rescales
  * the state to unit norm at the end, the norm accumulated in FP64 (the
  * precision form's renormalisation for complex64 states).  *npasses (may be

@@ -421,7 +421,10 @@ template <class T> struct Engine {
     // measured on MI355X (the 50-layer random circuit): complex128 at 30 qubits
     // is fastest with 2^10-amplitude tiles and 3 low bits, complex64 at 33
     // qubits with 2^11 and 4 (register pressure bounds larger tiles)
-    static constexpr int kTileDefaultBits = std::is_same_v<T, double> ? 10 : 11;
+    // defaults measured on the 50-layer random circuit (30 qubits complex128,
+    // 33 qubits complex64): the largest tile each precision's registers hold
+    // wins (fewer passes outweigh the lower occupancy)
+    static constexpr int kTileDefaultBits = std::is_same_v<T, double> ? 11 : 12;
     static constexpr int kTileMaxBitsC128 = 11;
     static constexpr int kTileDefaultLow = std::is_same_v<T, double> ? 3 : 4;
 
