@@ -2,7 +2,8 @@
 //
 // A pass fixes a set of k "tile bits": the `low` lowest bit positions
 // (contiguous runs of 2^low amplitudes in HBM) plus k - low higher positions
-// hb[0] < hb[1] < ...  Every assignment of the other n - k bits selects one
+// hb[0], hb[1], ... (in any order: the pass's plan puts first the ones that
+// become lane bits).  Every assignment of the other n - k bits selects one
 // tile of 2^k amplitudes (local index e: bits 0..low-1 of e are the
 // amplitude's bits 0..low-1, bit low + j of e is its bit hb[j]).  One
 // wavefront loads a tile, applies every operation of the pass -- each lowered
@@ -62,8 +63,8 @@ template <class T> struct TpOp {
 struct TpGeom {
     int k;                 // tile bits
     int low;               // local bits 0..low-1 = amplitude bits 0..low-1
-    int hb[kTpMaxBits];    // amplitude bit of local bit low + j (ascending)
-    Geom outer;            // pins hb[]: tile index << low -> base index
+    int hb[kTpMaxBits];    // amplitude bit of local bit low + j
+    Geom outer;            // pins hb[] (ascending): tile index << low -> base index
 };
 
 template <class T, int M>

@@ -456,7 +456,10 @@ template <class T> struct Engine {
         if (low < 0) {
             low = std::min(kTileDefaultLow, k - 6);
         }
-        const auto passes = tile_schedule(n(), t, k, low);
+        auto passes = tile_schedule(n(), t, k, low);
+        for (auto &ps : passes) { // lane bits where the pass's operations are cheapest
+            ps.hb = tile_lane_order<T>(n(), t, ps, low, woff, poff);
+        }
         if (std::is_same_v<T, double> && k > kTileMaxBitsC128) {
             throw InvalidArg("complex128 tiles hold at most 2^" + std::to_string(kTileMaxBitsC128) +
                              " amplitudes (a wavefront's registers)");
@@ -525,9 +528,11 @@ template <class T> struct Engine {
             g.low = low;
             g.outer.nb = static_cast<int>(ps.hb.size());
             g.outer.fixed = 0;
+            std::vector<int> asc = ps.hb;
+            std::sort(asc.begin(), asc.end());
             for (size_t j = 0; j < ps.hb.size(); j++) {
                 g.hb[j] = ps.hb[j];
-                g.outer.pos[j] = ps.hb[j];
+                g.outer.pos[j] = asc[j];
             }
             const TrInst<T> *pp = dprog + range[p].first;
             const int ni = range[p].second;

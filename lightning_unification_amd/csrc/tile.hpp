@@ -1,18 +1,21 @@
 // Tile-pass scheduling (host only): groups the operations of a tape into
 // passes of k_tile_pass.
 //
-// A pass owns the `low` lowest bit positions and up to k - low higher ones.
-// Operations are taken in tape order; an operation joins the open pass when
+// A pass owns the `low` lowest bit positions and k - low higher ones.  Given
+// those high bits, operations are taken in tape order; an operation joins the
+// pass when
 //   - it lowers to a group op of at most kTpMaxM amplitudes and kTpMaxPins
 //     pins (or to a parity phase),
-//   - its high bits fit in the pass's remaining budget, and
+//   - its high bits are among the pass's, and
 //   - it shares no bit with any earlier operation left out of the pass.
 // The last rule makes the reordering exact: an operation only moves ahead of
 // operations on disjoint bits, with which it commutes.  Operations left out
 // keep their relative order and are scheduled into later passes.  An
 // operation that can never be tiled (too many amplitudes per group) becomes a
-// pass of its own, applied by the per-gate kernels.  The high bits are padded
-// with the lowest unused positions, which lengthens the contiguous runs.
+// pass of its own, applied by the per-gate kernels.  The high bits start as
+// those the operations claim in tape order (padded with the lowest unused
+// positions) and are then improved by exchanging one bit at a time while the
+// pass takes more operations.
 #pragma once
 
 #include <algorithm>
@@ -26,10 +29,11 @@
 namespace plamd {
 
 constexpr size_t kTileLookahead = 4096; // operations scanned per pass
+constexpr int kTileSearchRounds = 16;    // bit exchanges tried per pass, at most
 
 struct TilePass {
     bool tiled = true;        // false: ops[0] applied by the per-gate kernels
-    std::vector<int> hb;      // bit positions of local bits low.. (ascending)
+    std::vector<int> hb;      // bit positions of local bits low.. (ascending; tile_lane_order reorders them)
     std::vector<int> ops;     // tape op indices in application order
 };
 
@@ -94,35 +98,89 @@ inline std::vector<TilePass> tile_schedule(int n, const TapeView &t, int k, int 
     }
     std::vector<TilePass> out;
     std::vector<int> rest;
-    while (!pending.empty()) {
-        TilePass p;
-        uint64_t hmask = 0, blocked = 0;
-        rest.clear();
+    // the pending operations a pass with high bits `hmask` takes, in order
+    // (each op whose high bits lie in hmask and which shares no bit with an
+    // op left out before it); fills `take` and `rest` when given
+    auto absorb = [&](uint64_t hmask, std::vector<int> *take, std::vector<int> *left) {
+        uint64_t blocked = 0;
+        int taken = 0;
         size_t j = 0;
         for (; j < pending.size() && j < kTileLookahead; j++) {
             const int i = pending[j];
-            const uint64_t hi = bits[i] & ~low_mask;
-            if (ok[i] && !(bits[i] & blocked) && __builtin_popcountll(hmask | hi) <= budget) {
-                hmask |= hi;
-                p.ops.push_back(i);
+            if (ok[i] && !(bits[i] & blocked) && ((bits[i] & ~low_mask) & ~hmask) == 0) {
+                taken++;
+                if (take) {
+                    take->push_back(i);
+                }
             } else {
                 blocked |= bits[i];
-                rest.push_back(i);
+                if (left) {
+                    left->push_back(i);
+                }
                 if (blocked == all) {
                     j++;
                     break;
                 }
             }
         }
-        rest.insert(rest.end(), pending.begin() + static_cast<long>(j), pending.end());
+        if (left) {
+            left->insert(left->end(), pending.begin() + static_cast<long>(j), pending.end());
+        }
+        return taken;
+    };
+    while (!pending.empty()) {
+        TilePass p;
+        // first guess: high bits claimed in tape order until the budget is spent
+        uint64_t hmask = 0, blocked = 0;
+        for (size_t j = 0; j < pending.size() && j < kTileLookahead && blocked != all; j++) {
+            const int i = pending[j];
+            const uint64_t hi = bits[i] & ~low_mask;
+            if (ok[i] && !(bits[i] & blocked) && __builtin_popcountll(hmask | hi) <= budget) {
+                hmask |= hi;
+            } else {
+                blocked |= bits[i];
+            }
+        }
+        for (int b = low; b < n && __builtin_popcountll(hmask) < budget; b++) {
+            hmask |= uint64_t{1} << b;
+        }
+        // then exchange one high bit for another while that lets the pass take
+        // more operations (a bit claimed early can cost the pass many later
+        // ones); on a 50-layer random circuit of 30 qubits this takes the
+        // passes from 81 to 62 (2^11-amplitude tiles)
+        int best = absorb(hmask, nullptr, nullptr);
+        for (int round = 0; round < kTileSearchRounds && best > 0; round++) {
+            uint64_t best_mask = hmask;
+            int best_n = best;
+            for (int a = low; a < n; a++) {
+                if (!((hmask >> a) & 1U)) {
+                    continue;
+                }
+                for (int b = low; b < n; b++) {
+                    if ((hmask >> b) & 1U) {
+                        continue;
+                    }
+                    const uint64_t m = (hmask & ~(uint64_t{1} << a)) | (uint64_t{1} << b);
+                    const int c = absorb(m, nullptr, nullptr);
+                    if (c > best_n) {
+                        best_n = c;
+                        best_mask = m;
+                    }
+                }
+            }
+            if (best_n == best) {
+                break;
+            }
+            best = best_n;
+            hmask = best_mask;
+        }
+        rest.clear();
+        absorb(hmask, &p.ops, &rest);
         if (p.ops.empty()) { // the first pending op cannot be tiled
             p.tiled = false;
             p.ops.push_back(pending[0]);
             rest.erase(rest.begin());
         } else {
-            for (int b = low; b < n && __builtin_popcountll(hmask) < budget; b++) {
-                hmask |= uint64_t{1} << b;
-            }
             for (int b = low; b < n; b++) {
                 if ((hmask >> b) & 1U) {
                     p.hb.push_back(b);
@@ -214,17 +272,75 @@ template <class T> int single_bit(const GroupOp<T> &g, std::complex<T> (&m)[4]) 
     return -1;
 }
 
+// Pinned and member bits of a group operation, and whether its fixed bits
+// are all pinned non-member bits (the forms the tile program applies).
+template <class T> bool tile_clean(const GroupOp<T> &g, uint32_t &pins, uint32_t &xo) {
+    pins = xo = 0;
+    for (int b = 0; b < g.nb; b++) {
+        pins |= 1U << g.pos[b];
+    }
+    for (int b = 0; b < g.M; b++) {
+        xo |= static_cast<uint32_t>(g.off[b]);
+    }
+    const uint32_t fx = static_cast<uint32_t>(g.fixed);
+    return (fx & ~pins) == 0 && (fx & xo) == 0 && (xo & ~pins) == 0;
+}
+
+// A diagonal group operation pinning at most two bits (its phases fold into
+// one table over those bits).
+template <class T> bool tile_small_diag(const GroupOp<T> &g) {
+    uint32_t pins, xo;
+    return g.kind == GroupOp<T>::Group && (g.M == 1 || g.diag) && g.M <= 4 && tile_clean(g, pins, xo) &&
+           __builtin_popcount(pins) <= 2;
+}
+
+// The phase such an operation applies to local index i (1 outside its groups).
+template <class T> std::complex<T> tile_diag_phase(const GroupOp<T> &g, uint32_t i) {
+    uint32_t pins, xo;
+    tile_clean(g, pins, xo);
+    const uint32_t cmask = pins & ~xo;
+    if ((i & cmask) != (static_cast<uint32_t>(g.fixed) & cmask)) {
+        return 1;
+    }
+    for (int j = 0; j < g.M; j++) {
+        if (static_cast<uint32_t>(g.off[j]) == (i & xo)) {
+            return g.mat[j * g.M + j];
+        }
+    }
+    return 1;
+}
+
 // The pass's operations lowered to the tile's local bits, in application
-// order, with single-bit operations on the same bit merged: an operation on
-// bit b whose last predecessor touching b is also a single-bit operation on b
-// is multiplied into it (the operations between touch other bits and commute
-// with both).  A random circuit's pass loses about a fifth of its operations.
+// order, with two kinds of merges:
+//  - single-bit operations on the same bit: an operation on bit b whose last
+//    predecessor touching b is also a single-bit operation on b is multiplied
+//    into it (the operations between touch other bits and commute with both);
+//  - diagonal operations: one on at most two bits is multiplied into the
+//    last earlier diagonal one it can move back to (every operation between
+//    them that touches its bits is diagonal too, so all of them commute)
+//    when the two pin at most two bits together.
+// A random circuit's pass loses about a fifth of its operations to the first
+// rule and a sixth of its diagonal ones to the second.
 template <class T>
 std::vector<GroupOp<T>> tile_ops_merged(int n, const TapeView &t, const TilePass &ps, int low,
                                         const std::vector<int> &woff, const std::vector<int> &poff) {
     const int k = low + static_cast<int>(ps.hb.size());
     std::vector<GroupOp<T>> out;
     std::vector<int> last(k, -1); // index in `out` of the last op touching each local bit
+    auto touch_of = [](const GroupOp<T> &g) {
+        uint64_t touch = 0;
+        if (g.kind == GroupOp<T>::Parity) {
+            touch = g.parity_mask;
+        } else {
+            for (int j = 0; j < g.nb; j++) {
+                touch |= uint64_t{1} << g.pos[j];
+            }
+        }
+        return touch;
+    };
+    auto diagonal = [](const GroupOp<T> &g) {
+        return g.kind == GroupOp<T>::Parity || (g.kind == GroupOp<T>::Group && (g.M == 1 || g.diag));
+    };
     for (int i : ps.ops) {
         auto g = tile_group<T>(n, t, i, woff[i], poff[i], low, ps.hb);
         if (g.kind == GroupOp<T>::Noop) {
@@ -240,12 +356,43 @@ std::vector<GroupOp<T>> tile_ops_merged(int n, const TapeView &t, const TilePass
             out[last[b]] = make_group<T>({b}, 0, {0, uint64_t{1} << b}, std::move(mm), dg);
             continue;
         }
-        uint64_t touch = 0;
-        if (g.kind == GroupOp<T>::Parity) {
-            touch = g.parity_mask;
-        } else {
-            for (int j = 0; j < g.nb; j++) {
-                touch |= uint64_t{1} << g.pos[j];
+        const uint64_t touch = touch_of(g);
+        if (tile_small_diag(g)) {
+            int into = -1;
+            for (int j = static_cast<int>(out.size()) - 1; j >= 0; j--) {
+                const uint64_t tj = touch_of(out[j]);
+                if (tile_small_diag(out[j]) && __builtin_popcountll(tj | touch) <= 2) {
+                    into = j;
+                    break;
+                }
+                if ((tj & touch) && !diagonal(out[j])) {
+                    break;
+                }
+            }
+            if (into >= 0) {
+                // one table over the union of the two operations' bits
+                const uint64_t u = touch_of(out[into]) | touch;
+                std::vector<int> bits;
+                for (int j = 0; j < k; j++) {
+                    if ((u >> j) & 1U) {
+                        bits.push_back(j);
+                    }
+                }
+                const int M = 1 << bits.size();
+                std::vector<uint64_t> off(M, 0);
+                std::vector<std::complex<T>> mm(static_cast<size_t>(M) * M, std::complex<T>(0));
+                for (int c = 0; c < M; c++) {
+                    for (size_t a = 0; a < bits.size(); a++) {
+                        off[c] |= ((c >> a) & 1) ? uint64_t{1} << bits[a] : 0;
+                    }
+                    const uint32_t li = static_cast<uint32_t>(off[c]);
+                    mm[c * M + c] = tile_diag_phase(g, li) * tile_diag_phase(out[into], li);
+                }
+                out[into] = make_group<T>(bits, 0, std::move(off), std::move(mm), true);
+                for (int j : bits) {
+                    last[j] = std::max(last[j], into);
+                }
+                continue;
             }
         }
         out.push_back(std::move(g));
@@ -256,6 +403,94 @@ std::vector<GroupOp<T>> tile_ops_merged(int n, const TapeView &t, const TilePass
         }
     }
     return out;
+}
+
+// Relative cost of an operation in a tile program when the local bits in
+// `lanes` are the lane bits, following the choices of tile_program.  Weights:
+// the per-operation times fitted to a pass over 2^30 complex128 amplitudes
+// in 2^11-amplitude tiles (ms): diagonal 0.15, 2x2 on a register bit 0.12,
+// register pair 0.11, 4x4 on two register bits 0.54, 2x2 on a lane bit 0.15,
+// lane x register 4x4 0.43, 4x4 on two lane bits 0.55; LDS about 1.
+template <class T> double tile_op_cost(const GroupOp<T> &g, uint32_t lanes) {
+    if (g.kind == GroupOp<T>::Parity) {
+        return 0.15;
+    }
+    uint32_t pins, xo;
+    const bool clean = tile_clean(g, pins, xo);
+    const int nx = __builtin_popcount(xo);
+    if (clean && (g.M == 1 || g.diag) && g.M <= 4 && nx <= 2) {
+        return 0.15;
+    }
+    const bool p1 = clean && g.M == 2 && g.off[0] == 0 && nx == 1;
+    const bool p2 = clean && g.M == 2 && nx == 2 && __builtin_popcountll(g.off[0]) == 1 &&
+                    __builtin_popcountll(g.off[1]) == 1;
+    const bool p4 = clean && g.M == 4 && g.off[0] == 0 && nx == 2 && __builtin_popcountll(g.off[1]) == 1 &&
+                    __builtin_popcountll(g.off[2]) == 1 && g.off[3] == (g.off[1] | g.off[2]);
+    const bool lane_cond = (pins & ~xo & lanes) != 0;
+    if ((p1 || p2 || (p4 && !lane_cond)) && (xo & lanes) == 0) {
+        return p1 ? 0.12 : (p2 ? 0.11 : 0.54);
+    }
+    if (p1) {
+        return 0.15;
+    }
+    if (clean && g.kind == GroupOp<T>::Group && g.M <= 4 && nx == 2 && (xo & lanes) != 0) {
+        return __builtin_popcount(xo & lanes) == 1 ? 0.43 : 0.55;
+    }
+    return 1.0;
+}
+
+// The order of a tiled pass's high bits that gives the cheapest program: the
+// 6 - low lane bits above the low ones are the high bits that minimise the
+// summed tile_op_cost (ties: the lowest such bits), then the register bits,
+// each part ascending.  Every order is exact (the local bits only place the
+// amplitudes in the wavefront tile); the search is skipped where it would
+// weigh more than 512 choices.
+template <class T>
+std::vector<int> tile_lane_order(int n, const TapeView &t, const TilePass &ps, int low, const std::vector<int> &woff,
+                                 const std::vector<int> &poff) {
+    const int nh = static_cast<int>(ps.hb.size()), nfree = std::max(0, 6 - low);
+    if (!ps.tiled || nfree == 0 || nfree >= nh) {
+        return ps.hb;
+    }
+    double choices = 1;
+    for (int i = 0; i < nfree; i++) {
+        choices = choices * (nh - i) / (i + 1);
+    }
+    if (choices > 512) {
+        return ps.hb;
+    }
+    const auto ops = tile_ops_merged<T>(n, t, ps, low, woff, poff);
+    const uint32_t lowlanes = (1U << low) - 1;
+    uint32_t best_sel = (1U << nfree) - 1;
+    double best = 0;
+    for (const auto &g : ops) {
+        best += tile_op_cost(g, lowlanes | (best_sel << low));
+    }
+    for (uint32_t sel = 0; sel < (1U << nh); sel++) {
+        if (__builtin_popcount(sel) != nfree || sel == ((1U << nfree) - 1)) {
+            continue;
+        }
+        double c = 0;
+        for (const auto &g : ops) {
+            c += tile_op_cost(g, lowlanes | (sel << low));
+        }
+        if (c < best - 1e-9) {
+            best = c;
+            best_sel = sel;
+        }
+    }
+    std::vector<int> hb;
+    for (int j = 0; j < nh; j++) {
+        if ((best_sel >> j) & 1U) {
+            hb.push_back(ps.hb[j]);
+        }
+    }
+    for (int j = 0; j < nh; j++) {
+        if (!((best_sel >> j) & 1U)) {
+            hb.push_back(ps.hb[j]);
+        }
+    }
+    return hb;
 }
 
 template <class T>
