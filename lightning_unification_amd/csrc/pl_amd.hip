@@ -418,9 +418,6 @@ template <class T> struct Engine {
     }
 
     // ---- tile passes: many gates per read and write of the state --------------
-    // measured on MI355X (the 50-layer random circuit): complex128 at 30 qubits
-    // is fastest with 2^10-amplitude tiles and 3 low bits, complex64 at 33
-    // qubits with 2^11 and 4 (register pressure bounds larger tiles)
     // defaults measured on the 50-layer random circuit (30 qubits complex128,
     // 33 qubits complex64): the largest tile each precision's registers hold
     // wins (fewer passes outweigh the lower occupancy)
@@ -428,10 +425,11 @@ template <class T> struct Engine {
     static constexpr int kTileMaxBitsC128 = 11;
     static constexpr int kTileDefaultLow = std::is_same_v<T, double> ? 3 : 4;
 
-
     // Applies a tape as tile passes (tile.hpp); returns the number of launches.
     // k = 0 / low < 0 pick the measured defaults of the precision; with k = 0
-    // a state narrower than the smallest tile runs gate by gate.
+    // a state narrower than the smallest tile runs gate by gate.  Each pass is
+    // planned, compiled and launched before the next is planned, so the host's
+    // planning overlaps the device's passes.
     int apply_tiled(V *sv, const TapeView &t, int k, int low) {
         std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
         for (int i = 0; i < t.nops; i++) {
@@ -456,10 +454,7 @@ template <class T> struct Engine {
         if (low < 0) {
             low = std::min(kTileDefaultLow, k - 6);
         }
-        auto passes = tile_schedule(n(), t, k, low);
-        for (auto &ps : passes) { // lane bits where the pass's operations are cheapest
-            ps.hb = tile_lane_order<T>(n(), t, ps, low, woff, poff);
-        }
+        TileScheduler sched(n(), t, k, low);
         if (std::is_same_v<T, double> && k > kTileMaxBitsC128) {
             throw InvalidArg("complex128 tiles hold at most 2^" + std::to_string(kTileMaxBitsC128) +
                              " amplitudes (a wavefront's registers)");
@@ -470,58 +465,54 @@ template <class T> struct Engine {
         if ((size_t{1} << k) * sizeof(V) > static_cast<size_t>(lds_max)) {
             throw InvalidArg("a tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS of a workgroup");
         }
-        // every pass's program in one device buffer, the LDS-applied ops after them
+        // the passes' programs, uploaded pass by pass into one device buffer
+        // (instructions, then the LDS-applied ops): a program holds at most one
+        // instruction and one LDS op per operation, so the tape's length bounds
+        // both, and the host copies never move while a copy may read them
+        const size_t cap = static_cast<size_t>(std::max(t.nops, 1));
         std::vector<TrInst<T>> prog;
         std::vector<TpOp<T>> lops;
-        std::vector<std::pair<size_t, int>> range(passes.size(), {0, 0});
-        std::vector<char> uses_lds(passes.size(), 0);
-        for (size_t p = 0; p < passes.size(); p++) {
-            if (!passes[p].tiled) {
-                continue;
-            }
-            range[p].first = prog.size();
-            const size_t nl = lops.size();
-            tile_program<T>(n(), t, passes[p], low, woff, poff, prog, lops);
-            range[p].second = static_cast<int>(prog.size() - range[p].first);
-            uses_lds[p] = lops.size() > nl;
-        }
-        const size_t pbytes = (prog.size() * sizeof(TrInst<T>) + 255) & ~size_t{255};
-        const size_t bytes = pbytes + lops.size() * sizeof(TpOp<T>);
+        prog.reserve(cap);
+        lops.reserve(cap);
+        const size_t pbytes = (cap * sizeof(TrInst<T>) + 255) & ~size_t{255};
         unsigned char *dev = nullptr;
-        if (bytes) {
-            HIP_OK(hipMalloc(&dev, bytes));
-        }
+        HIP_OK(hipMalloc(&dev, pbytes + cap * sizeof(TpOp<T>)));
         struct Free {
             void *p;
             hipStream_t s;
             ~Free() {
-                if (p) {
-                    (void)hipStreamSynchronize(s);
-                    (void)hipFree(p);
-                }
+                (void)hipStreamSynchronize(s);
+                (void)hipFree(p);
             }
         } guard{dev, h->stream};
-        if (!prog.empty()) {
-            HIP_OK(hipMemcpyAsync(dev, prog.data(), prog.size() * sizeof(TrInst<T>), hipMemcpyHostToDevice,
-                                  h->stream));
-        }
-        if (!lops.empty()) {
-            HIP_OK(hipMemcpyAsync(dev + pbytes, lops.data(), lops.size() * sizeof(TpOp<T>),
-                                  hipMemcpyHostToDevice, h->stream));
-        }
-        const auto *dprog = reinterpret_cast<const TrInst<T> *>(dev);
-        const auto *dlops = reinterpret_cast<const TpOp<T> *>(dev + pbytes);
+        auto *dprog = reinterpret_cast<TrInst<T> *>(dev);
+        auto *dlops = reinterpret_cast<TpOp<T> *>(dev + pbytes);
         const uint64_t ntiles = uint64_t{1} << (n() - k);
-        for (size_t p = 0; p < passes.size(); p++) {
-            const auto &ps = passes[p];
+        int npasses = 0;
+        TilePass ps;
+        while (sched.next(ps)) {
+            npasses++;
             if (!ps.tiled) {
                 const int o = ps.ops[0];
                 gate(sv, t.ids[o], t.wires + woff[o], t.nwires[o], t.inverse[o] != 0, t.params + poff[o],
                      t.nparams[o]);
                 continue;
             }
-            if (range[p].second == 0) {
+            ps.hb = tile_lane_order<T>(n(), t, ps, low, woff, poff); // lane bits where its ops are cheapest
+            const size_t p0 = prog.size(), l0 = lops.size();
+            tile_program<T>(n(), t, ps, low, woff, poff, prog, lops);
+            if (prog.size() > cap || lops.size() > cap) {
+                throw std::logic_error("tile programs longer than their tape");
+            }
+            const int ni = static_cast<int>(prog.size() - p0);
+            if (ni == 0) {
                 continue;
+            }
+            HIP_OK(hipMemcpyAsync(dprog + p0, prog.data() + p0, ni * sizeof(TrInst<T>), hipMemcpyHostToDevice,
+                                  h->stream));
+            if (lops.size() > l0) {
+                HIP_OK(hipMemcpyAsync(dlops + l0, lops.data() + l0, (lops.size() - l0) * sizeof(TpOp<T>),
+                                      hipMemcpyHostToDevice, h->stream));
             }
             TpGeom g{};
             g.k = k;
@@ -534,13 +525,10 @@ template <class T> struct Engine {
                 g.hb[j] = ps.hb[j];
                 g.outer.pos[j] = asc[j];
             }
-            const TrInst<T> *pp = dprog + range[p].first;
-            const int ni = range[p].second;
-            const bool ul = uses_lds[p] != 0;
-            HIP_OK(launch_tile_pass<T>(k - 6, sv, ntiles, g, pp, ni, dlops, ul, h->stream));
+            HIP_OK(launch_tile_pass<T>(k - 6, sv, ntiles, g, dprog + p0, ni, dlops, lops.size() > l0, h->stream));
         }
         HIP_OK(hipStreamSynchronize(h->stream));
-        return static_cast<int>(passes.size());
+        return npasses;
     }
 
     // ---- reductions ----------------------------------------------------------

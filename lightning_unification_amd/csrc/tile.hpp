@@ -14,8 +14,8 @@
 // operation that can never be tiled (too many amplitudes per group) becomes a
 // pass of its own, applied by the per-gate kernels.  The high bits start as
 // those the operations claim in tape order (padded with the lowest unused
-// positions) and are then improved by exchanging one bit at a time while the
-// pass takes more operations.
+// positions) and are then improved by exchanging bits while the pass takes
+// more operations.
 #pragma once
 
 #include <algorithm>
@@ -30,6 +30,7 @@ namespace plamd {
 
 constexpr size_t kTileLookahead = 4096; // operations scanned per pass
 constexpr int kTileSearchRounds = 16;    // bit exchanges tried per pass, at most
+constexpr int kTileDeepSearchQubits = 30; // states from which pair exchanges are tried too
 
 struct TilePass {
     bool tiled = true;        // false: ops[0] applied by the per-gate kernels
@@ -65,98 +66,76 @@ inline void tile_check_args(int n, int k, int low) {
     }
 }
 
-inline std::vector<TilePass> tile_schedule(int n, const TapeView &t, int k, int low) {
-    tile_check_args(n, k, low);
-    const int budget = k - low;
-    std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
-    for (int i = 0; i < t.nops; i++) {
-        if (t.nwires[i] < 0 || t.nparams[i] < 0) {
-            throw InvalidArg("negative wire or parameter count");
-        }
-        woff[i + 1] = woff[i] + t.nwires[i];
-        poff[i + 1] = poff[i] + t.nparams[i];
-    }
-    const uint64_t low_mask = (uint64_t{1} << low) - 1;
-    const uint64_t all = n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1;
-    std::vector<uint64_t> bits(t.nops, 0);
-    std::vector<char> ok(t.nops, 0);
-    std::vector<int> pending;
-    for (int i = 0; i < t.nops; i++) {
-        if (t.ids[i] == -1) {
-            continue; // state preparation: skipped, as by applyOperations
-        }
-        for (int a = 0; a < t.nwires[i]; a++) {
-            const int w = t.wires[woff[i] + a];
-            if (w < 0 || w >= n) {
-                throw InvalidArg("wire index out of range");
+// The passes of a tape, one at a time (the executor plans a pass while the
+// device applies the one before).
+class TileScheduler {
+  public:
+    TileScheduler(int n, const TapeView &t, int k, int low)
+        : n_((tile_check_args(n, k, low), n)), low_(low), budget_(k - low), low_mask_((uint64_t{1} << low) - 1),
+          all_(n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1), deep_(n >= kTileDeepSearchQubits),
+          bits_(t.nops, 0), ok_(t.nops, 0) {
+        std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
+        for (int i = 0; i < t.nops; i++) {
+            if (t.nwires[i] < 0 || t.nparams[i] < 0) {
+                throw InvalidArg("negative wire or parameter count");
             }
-            bits[i] |= uint64_t{1} << (n - 1 - w);
+            woff[i + 1] = woff[i] + t.nwires[i];
+            poff[i + 1] = poff[i] + t.nparams[i];
         }
-        ok[i] = tile_op_ok(t.ids[i], t.nwires[i], t.inverse[i] != 0, t.params + poff[i], t.nparams[i]) &&
-                __builtin_popcountll(bits[i] & ~low_mask) <= budget;
-        pending.push_back(i);
-    }
-    std::vector<TilePass> out;
-    std::vector<int> rest;
-    // the pending operations a pass with high bits `hmask` takes, in order
-    // (each op whose high bits lie in hmask and which shares no bit with an
-    // op left out before it); fills `take` and `rest` when given
-    auto absorb = [&](uint64_t hmask, std::vector<int> *take, std::vector<int> *left) {
-        uint64_t blocked = 0;
-        int taken = 0;
-        size_t j = 0;
-        for (; j < pending.size() && j < kTileLookahead; j++) {
-            const int i = pending[j];
-            if (ok[i] && !(bits[i] & blocked) && ((bits[i] & ~low_mask) & ~hmask) == 0) {
-                taken++;
-                if (take) {
-                    take->push_back(i);
-                }
-            } else {
-                blocked |= bits[i];
-                if (left) {
-                    left->push_back(i);
-                }
-                if (blocked == all) {
-                    j++;
-                    break;
-                }
+        for (int i = 0; i < t.nops; i++) {
+            if (t.ids[i] == -1) {
+                continue; // state preparation: skipped, as by applyOperations
             }
+            for (int a = 0; a < t.nwires[i]; a++) {
+                const int w = t.wires[woff[i] + a];
+                if (w < 0 || w >= n) {
+                    throw InvalidArg("wire index out of range");
+                }
+                bits_[i] |= uint64_t{1} << (n - 1 - w);
+            }
+            ok_[i] = tile_op_ok(t.ids[i], t.nwires[i], t.inverse[i] != 0, t.params + poff[i], t.nparams[i]) &&
+                     __builtin_popcountll(bits_[i] & ~low_mask_) <= budget_;
+            pending_.push_back(i);
         }
-        if (left) {
-            left->insert(left->end(), pending.begin() + static_cast<long>(j), pending.end());
+    }
+
+    // The next pass (false once every operation is scheduled).
+    bool next(TilePass &p) {
+        if (pending_.empty()) {
+            return false;
         }
-        return taken;
-    };
-    while (!pending.empty()) {
-        TilePass p;
+        p = TilePass{};
         // first guess: high bits claimed in tape order until the budget is spent
         uint64_t hmask = 0, blocked = 0;
-        for (size_t j = 0; j < pending.size() && j < kTileLookahead && blocked != all; j++) {
-            const int i = pending[j];
-            const uint64_t hi = bits[i] & ~low_mask;
-            if (ok[i] && !(bits[i] & blocked) && __builtin_popcountll(hmask | hi) <= budget) {
+        for (size_t j = 0; j < pending_.size() && j < kTileLookahead && blocked != all_; j++) {
+            const int i = pending_[j];
+            const uint64_t hi = bits_[i] & ~low_mask_;
+            if (ok_[i] && !(bits_[i] & blocked) && __builtin_popcountll(hmask | hi) <= budget_) {
                 hmask |= hi;
             } else {
-                blocked |= bits[i];
+                blocked |= bits_[i];
             }
         }
-        for (int b = low; b < n && __builtin_popcountll(hmask) < budget; b++) {
+        for (int b = low_; b < n_ && __builtin_popcountll(hmask) < budget_; b++) {
             hmask |= uint64_t{1} << b;
         }
-        // then exchange one high bit for another while that lets the pass take
+        // then exchange high bits for others while that lets the pass take
         // more operations (a bit claimed early can cost the pass many later
-        // ones); on a 50-layer random circuit of 30 qubits this takes the
-        // passes from 81 to 62 (2^11-amplitude tiles)
+        // ones): the best single exchange each round, or failing one (on
+        // states of kTileDeepSearchQubits or more, where a pass takes the
+        // device longer than this search takes the host) the first pair
+        // exchange that helps.  On a 50-layer random circuit of 30 qubits
+        // (2^11-amplitude tiles) the passes go from 81 to 62 with single
+        // exchanges and to 55 with pairs.
         int best = absorb(hmask, nullptr, nullptr);
         for (int round = 0; round < kTileSearchRounds && best > 0; round++) {
             uint64_t best_mask = hmask;
             int best_n = best;
-            for (int a = low; a < n; a++) {
+            for (int a = low_; a < n_; a++) {
                 if (!((hmask >> a) & 1U)) {
                     continue;
                 }
-                for (int b = low; b < n; b++) {
+                for (int b = low_; b < n_; b++) {
                     if ((hmask >> b) & 1U) {
                         continue;
                     }
@@ -168,27 +147,105 @@ inline std::vector<TilePass> tile_schedule(int n, const TapeView &t, int k, int 
                     }
                 }
             }
+            if (best_n == best && deep_) {
+                best_mask = pair_exchange(hmask, best, best_n);
+            }
             if (best_n == best) {
                 break;
             }
             best = best_n;
             hmask = best_mask;
         }
-        rest.clear();
+        std::vector<int> rest;
         absorb(hmask, &p.ops, &rest);
         if (p.ops.empty()) { // the first pending op cannot be tiled
             p.tiled = false;
-            p.ops.push_back(pending[0]);
+            p.ops.push_back(pending_[0]);
             rest.erase(rest.begin());
         } else {
-            for (int b = low; b < n; b++) {
+            for (int b = low_; b < n_; b++) {
                 if ((hmask >> b) & 1U) {
                     p.hb.push_back(b);
                 }
             }
         }
+        pending_.swap(rest);
+        return true;
+    }
+
+  private:
+    // The pending operations a pass with high bits `hmask` takes, in order:
+    // each op whose high bits lie in hmask and which shares no bit with an op
+    // left out before it.  Fills `take` and `left` when given.
+    int absorb(uint64_t hmask, std::vector<int> *take, std::vector<int> *left) const {
+        uint64_t blocked = 0;
+        int taken = 0;
+        size_t j = 0;
+        for (; j < pending_.size() && j < kTileLookahead; j++) {
+            const int i = pending_[j];
+            if (ok_[i] && !(bits_[i] & blocked) && ((bits_[i] & ~low_mask_) & ~hmask) == 0) {
+                taken++;
+                if (take) {
+                    take->push_back(i);
+                }
+            } else {
+                blocked |= bits_[i];
+                if (left) {
+                    left->push_back(i);
+                }
+                if (blocked == all_) {
+                    j++;
+                    break;
+                }
+            }
+        }
+        if (left) {
+            left->insert(left->end(), pending_.begin() + static_cast<long>(j), pending_.end());
+        }
+        return taken;
+    }
+
+    // The first exchange of two high bits for two others that takes more
+    // than `best` operations (its count in best_n), else hmask unchanged.
+    uint64_t pair_exchange(uint64_t hmask, int best, int &best_n) const {
+        for (int a = low_; a < n_; a++) {
+            for (int a2 = a + 1; a2 < n_ && ((hmask >> a) & 1U); a2++) {
+                if (!((hmask >> a2) & 1U)) {
+                    continue;
+                }
+                for (int b = low_; b < n_; b++) {
+                    for (int b2 = b + 1; b2 < n_ && !((hmask >> b) & 1U); b2++) {
+                        if ((hmask >> b2) & 1U) {
+                            continue;
+                        }
+                        const uint64_t m = (hmask & ~(uint64_t{1} << a) & ~(uint64_t{1} << a2)) |
+                                           (uint64_t{1} << b) | (uint64_t{1} << b2);
+                        const int c = absorb(m, nullptr, nullptr);
+                        if (c > best) {
+                            best_n = c;
+                            return m;
+                        }
+                    }
+                }
+            }
+        }
+        return hmask;
+    }
+
+    int n_, low_, budget_;
+    uint64_t low_mask_, all_;
+    bool deep_;
+    std::vector<uint64_t> bits_;
+    std::vector<char> ok_;
+    std::vector<int> pending_;
+};
+
+inline std::vector<TilePass> tile_schedule(int n, const TapeView &t, int k, int low) {
+    TileScheduler s(n, t, k, low);
+    std::vector<TilePass> out;
+    TilePass p;
+    while (s.next(p)) {
         out.push_back(std::move(p));
-        pending.swap(rest);
     }
     return out;
 }
