@@ -73,6 +73,28 @@ def test_plan_reproduces_tape(oracle, n, k, low):
     assert rel_err(got, exp) < 1e-12
 
 
+def check_structure(n, k, low, tape, plan):
+    """Every operation once; tiled passes hold only operations inside their
+    tile; an operation overtakes only earlier operations on disjoint wires."""
+    prep = [i for i, op in enumerate(tape.ops) if op[0] not in GATE_ID]
+    order = [i for _, _, ops in plan for i in ops]
+    assert sorted(order) == [i for i in range(len(tape.ops)) if i not in prep]
+    low_mask = (1 << low) - 1
+    for tiled, hb, ops in plan:
+        if not tiled:
+            assert len(ops) == 1
+            continue
+        assert len(hb) == k - low and hb == sorted(hb) and all(low <= b < n for b in hb)
+        allowed = low_mask | sum(1 << b for b in hb)
+        for i in ops:
+            assert op_bits(n, tape.ops[i]) & ~allowed == 0, (tape.ops[i], hb)
+    bits = [op_bits(n, tape.ops[i]) for i in order]
+    for p, a in enumerate(order):
+        for q in range(p + 1, len(order)):
+            if order[q] < a:
+                assert bits[p] & bits[q] == 0
+
+
 def test_plan_counts_on_the_30_qubit_circuit():
     from lightning_unification_amd import capi
     tape = random_layers(30, 50, seed=1002)
@@ -81,6 +103,28 @@ def test_plan_counts_on_the_30_qubit_circuit():
     assert sum(len(ops) for _, _, ops in plan) == len(tape.ops)
     # the greedy schedule with commutation look-ahead packs ~10 gates per pass
     assert len(plan) <= len(tape.ops) // 8
+
+
+def test_search_packs_the_30_qubit_circuit():
+    """Each pass's high bits are searched (one-bit exchanges, and from 30
+    qubits two-bit exchanges): the 50-layer circuit of 990 gates fits in 55
+    passes of 2^11-amplitude tiles with 3 low bits, where claiming bits in
+    tape order took 81.  The plan keeps every structural rule."""
+    from lightning_unification_amd import capi
+    tape = random_layers(30, 50, seed=1002)
+    plan = capi.tile_plan(30, tape, 11, 3)
+    assert len(plan) <= 55
+    check_structure(30, 11, 3, tape, plan)
+
+
+@pytest.mark.parametrize("k,low", [(11, 3), (12, 4), (9, 0)])
+def test_search_keeps_the_rules_on_30_qubits(k, low):
+    """A tape over every gate kind on 30 qubits, where the two-bit exchanges
+    run: the plan still moves operations only past disjoint ones."""
+    from lightning_unification_amd import capi
+    tape = zoo_tape(30, 7 + k + low, count=400)
+    plan = capi.tile_plan(30, tape, k, low)
+    check_structure(30, k, low, tape, plan)
 
 
 def test_wide_operations_are_not_tiled():
