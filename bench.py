@@ -398,6 +398,18 @@ def bench_tiled(sv_ref, tape, n, reps):
         sv.synchronize()
         times.append(sv.event_elapsed(0, 1))
     dist = sv.distance(sv_ref)
+    # the same gates one pl_sv_apply_gate call each, deferred (what a caller
+    # applying gate by gate, like the drop-in module, gets): from the first
+    # call to the queue applied, once, on a fresh |0>
+    sv.reset()
+    sv.set_deferred(True)
+    sv.synchronize()
+    sv.event_record(0)
+    for name, wires, inv, params in tape.ops:
+        sv.apply(name, wires, inv, params)
+    sv.event_record(1)  # applies the queue first
+    ms_deferred = sv.event_elapsed(0, 1)
+    sv.set_deferred(False)
     sv.close()
     ms = min(times)
     out = {"passes": npass, "gates": len(tape), "ms_per_circuit": round(ms, 2),
@@ -405,8 +417,10 @@ def bench_tiled(sv_ref, tape, n, reps):
            "hbm_GBs": round(npass * 2 * S / (ms / 1e3) / 1e9, 1),
            "hbm_frac": round(npass * 2 * S / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
            "gate_equivalent_GBs": round(gate_bytes / (ms / 1e3) / 1e9, 1),
-           "l2_vs_per_gate": dist}
-    log(f"tiled: {npass} passes, {ms:.1f} ms per circuit, |dpsi| {dist:.2e}")
+           "l2_vs_per_gate": dist,
+           "deferred_gate_calls": {"calls": len(tape.ops), "ms": round(ms_deferred, 2)}}
+    log(f"tiled: {npass} passes, {ms:.1f} ms per circuit, |dpsi| {dist:.2e}; "
+        f"{len(tape.ops)} deferred gate calls {ms_deferred:.1f} ms")
     return out
 
 

@@ -105,6 +105,20 @@ int pl_sv_apply_gate(pl_sv_t sv, int gate_id, const int *wires, int nwires,
 int pl_sv_apply_ops(pl_sv_t sv, int nops, const int *ids, const int *nwires,
                     const int *wires, const uint8_t *inverse,
                     const int *nparams, const double *params);
+/* Deferred gates (no counterpart in the reference: its applyOperation
+ * returns with the gate applied; here the result is the same by the time
+ * anything can observe it).  With on != 0, pl_sv_apply_gate and
+ * pl_sv_apply_ops check each gate as they otherwise would (same errors, at
+ * the call) and queue it; the queue is applied as tile passes, with the
+ * precision's default tiles, before any other call on sv reads or changes
+ * the state (measurements, adjoint, copies, downloads, pl_sv_device_ptr,
+ * pl_sv_synchronize, timing events), and whenever it holds 4096 gates.  A
+ * call that overwrites the whole state (pl_sv_upload, pl_sv_set_basis_state,
+ * pl_sv_reset, pl_sv_copy into sv) drops it.  on = 0 applies the queue and
+ * returns to gate-by-gate application.  Off by default. */
+int pl_sv_set_deferred(pl_sv_t sv, int on);
+/* Gates queued on sv and not yet applied. */
+int pl_sv_deferred_count(pl_sv_t sv, int *out);
 /* StateVectorLQubit::applyMatrix(matrix, wires, inverse)
  * (StateVectorLQubit.hpp:329-347; LM.hpp:204-325, PI.hpp:250-290). */
 int pl_sv_apply_matrix(pl_sv_t sv, const double *matrix, const int *wires,

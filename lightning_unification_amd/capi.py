@@ -32,7 +32,7 @@ EXPORTS = [
     "pl_mlp_get_params", "pl_mlp_set_params", "pl_mlp_set_loss_weights", "pl_mlp_predict",
     "pl_mlp_gradients", "pl_mlp_load_data", "pl_mlp_train", "pl_vjp", "pl_expval_sparse",
     "pl_var_sparse", "pl_generate_mcmc_samples", "pl_sv_apply_ops_tiled", "pl_tile_plan",
-    "pl_transfer_counters", "pl_expval_noalloc",
+    "pl_transfer_counters", "pl_expval_noalloc", "pl_sv_set_deferred", "pl_sv_deferred_count",
 ]
 
 PL_FUSE_RENORM, PL_FUSE_NO_MFMA = 1, 2
@@ -182,6 +182,16 @@ class DeviceStateVector:
 
     def synchronize(self):
         _check(load().pl_sv_synchronize(self.h))
+
+    def set_deferred(self, on=True):
+        """Queue gates and apply them as tile passes when the state is next
+        used (pl_sv_set_deferred); on=False applies the queue."""
+        _check(load().pl_sv_set_deferred(self.h, int(bool(on))))
+
+    def deferred_count(self):
+        out = ctypes.c_int()
+        _check(load().pl_sv_deferred_count(self.h, ctypes.byref(out)))
+        return out.value
 
     # ---- gates ---------------------------------------------------------------
     def apply(self, name, wires, inverse=False, params=()):

@@ -225,6 +225,13 @@ struct pl_sv {
     size_t staging_bytes = 0;
     std::vector<hipEvent_t> slots; // pl_event_record slots
     std::vector<std::pair<void *, size_t>> bufs; // grow-only device buffers by purpose (plamd::Buf)
+    // gates queued while deferral is on (pl_sv_set_deferred), in the tape
+    // encoding of pl_sv_apply_ops; applied as tile passes before anything
+    // else reads or changes the state
+    bool defer = false;
+    std::vector<int> q_ids, q_nwires, q_wires, q_nparams;
+    std::vector<uint8_t> q_inverse;
+    std::vector<double> q_params;
 
     size_t state_bytes() const { return len * (dtype == PL_C128 ? 16 : 8); }
 };

@@ -109,6 +109,9 @@ template <class PrecisionT_> class StateVectorAMD {
     [[nodiscard]] size_t getNumQubits() const { return num_qubits_; }
     [[nodiscard]] size_t getLength() const { return size_t{1} << num_qubits_; }
     [[nodiscard]] pl_sv_t handle() const { return h_; }
+    /* queue gates and apply them as tile passes when the state is next used
+     * (pl_sv_set_deferred); off applies the queue */
+    void setDeferred(bool on) { check(pl_sv_set_deferred(h_, on ? 1 : 0)); }
 
     void applyOperation(const std::string &opName, const std::vector<size_t> &wires,
                         bool inverse = false, const std::vector<PrecisionT> &params = {}) {

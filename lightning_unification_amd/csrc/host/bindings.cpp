@@ -17,6 +17,10 @@
 //   - gates change the device copy only; the buffer is written back when a
 //     wrapper is destroyed (the device's `sim` at the end of
 //     apply_lightning) or on sync();
+//   - gates are deferred (pl_sv_set_deferred): the gates of apply_lightning,
+//     one call each, queue up and run as tile passes when the state is next
+//     used (a measurement, the adjoint, the write-back), so a circuit reads
+//     and writes the state once per pass instead of once per gate;
 //   - a call finding both the device copy changed by gates and the buffer
 //     written on the host raises instead of choosing one of them;
 //   - the mirror is freed when the buffer's owning array dies (a weak
@@ -165,6 +169,7 @@ template <class P> struct PyStateVector {
         mir->nbytes = key.second;
         mir->key = key;
         mir->sv = std::make_unique<SV>(static_cast<const std::complex<P> *>(info.ptr), len);
+        mir->sv->setDeferred(true); // a circuit's gates run as tile passes
         mir->watch.watch(info.ptr, key.second);
         // register under a weak reference to the buffer's owner
         py::object owner = buffer_owner(a);
@@ -180,6 +185,7 @@ template <class P> struct PyStateVector {
     }
     explicit PyStateVector(size_t num_qubits) : arr(py::none()), mir(std::make_shared<Mirror<P>>()) {
         mir->sv = std::make_unique<SV>(num_qubits);
+        mir->sv->setDeferred(true);
     }
     ~PyStateVector() {
         try {

@@ -1124,8 +1124,66 @@ template <class T> struct Engine {
     }
 };
 
+// ---- deferred gates --------------------------------------------------------
+// While deferral is on, pl_sv_apply_gate / pl_sv_apply_ops check each gate as
+// the per-gate path would (same errors, at the call) and queue it; the queue
+// is applied as tile passes (the precision's default tiles) by the first call
+// that reads or changes the state otherwise, or once it holds
+// kDeferMaxOps gates.
+constexpr size_t kDeferMaxOps = 4096;
+
+void flush_deferred(pl_sv_t h) {
+    if (!h || h->q_ids.empty()) {
+        return;
+    }
+    // the queue leaves the handle first: a failure cannot apply it twice
+    std::vector<int> ids, nw, wires, np;
+    std::vector<uint8_t> inv;
+    std::vector<double> prm;
+    ids.swap(h->q_ids);
+    nw.swap(h->q_nwires);
+    wires.swap(h->q_wires);
+    np.swap(h->q_nparams);
+    inv.swap(h->q_inverse);
+    prm.swap(h->q_params);
+    const TapeView t{static_cast<int>(ids.size()), ids.data(), nw.data(), wires.data(), inv.data(), np.data(),
+                     prm.data()};
+    HIP_OK(hipSetDevice(0));
+    if (h->dtype == PL_C128) {
+        Engine<double> e(h);
+        e.apply_tiled(e.psi(), t, 0, -1);
+    } else {
+        Engine<float> e(h);
+        e.apply_tiled(e.psi(), t, 0, -1);
+    }
+}
+
+// the queue dropped: the state is about to be overwritten whole
+void discard_deferred(pl_sv_t h) {
+    h->q_ids.clear();
+    h->q_nwires.clear();
+    h->q_wires.clear();
+    h->q_nparams.clear();
+    h->q_inverse.clear();
+    h->q_params.clear();
+}
+
+void defer_gate(pl_sv_t h, int id, const int *wires, int nw, bool inverse, const double *params, int np) {
+    (void)lower_gate<double>(h->n, id, wires, nw, inverse, params, np); // the per-gate path's checks
+    h->q_ids.push_back(id);
+    h->q_nwires.push_back(nw);
+    h->q_wires.insert(h->q_wires.end(), wires, wires + nw);
+    h->q_inverse.push_back(inverse ? 1 : 0);
+    h->q_nparams.push_back(np);
+    h->q_params.insert(h->q_params.end(), params, params + np);
+    if (h->q_ids.size() >= kDeferMaxOps) {
+        flush_deferred(h);
+    }
+}
+
 // Engine of the handle's precision, exceptions propagate
 template <class F> void dispatch_or_throw(pl_sv_t h, F &&f) {
+    flush_deferred(h);
     HIP_OK(hipSetDevice(0));
     if (h->dtype == PL_C128) {
         Engine<double> e(h);
@@ -1141,6 +1199,7 @@ template <class F> int dispatch(pl_sv_t h, F &&f) {
         if (!h) {
             throw InvalidArg("null state-vector handle");
         }
+        flush_deferred(h);
         HIP_OK(hipSetDevice(0));
         if (h->dtype == PL_C128) {
             Engine<double> e(h);
@@ -1694,6 +1753,7 @@ int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {
         if (nbytes != h->state_bytes()) {
             throw InvalidArg("upload size does not match the state vector size");
         }
+        discard_deferred(h);
         HIP_OK(hipMemcpyAsync(h->d, host, nbytes, hipMemcpyHostToDevice, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         g_h2d_bytes.fetch_add(nbytes, std::memory_order_relaxed);
@@ -1705,6 +1765,7 @@ int pl_sv_download(pl_sv_t h, void *host, size_t nbytes) {
         if (nbytes != h->state_bytes()) {
             throw InvalidArg("download size does not match the state vector size");
         }
+        flush_deferred(h);
         HIP_OK(hipMemcpyAsync(host, h->d, nbytes, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         g_d2h_bytes.fetch_add(nbytes, std::memory_order_relaxed);
@@ -1725,6 +1786,8 @@ int pl_sv_copy(pl_sv_t dst, pl_sv_t src) {
         if (dst == src) {
             return;
         }
+        flush_deferred(src);
+        discard_deferred(dst);
         // dst's stream waits for src's queued work, copies, and src's stream
         // then waits for the copy, so later work queued on src (a gate) cannot
         // overwrite amplitudes the copy has not read yet
@@ -1739,6 +1802,7 @@ int pl_sv_set_basis_state(pl_sv_t h, uint64_t index) {
     return guarded([&] {
         if (!h) throw InvalidArg("null argument");
         if (index >= h->len) throw InvalidArg("basis index out of range");
+        discard_deferred(h);
         const unsigned grid = grid_for(h->len);
         if (h->dtype == PL_C128) {
             hipLaunchKernelGGL((k_fill_basis<double>), dim3(grid), dim3(kBlock), 0, h->stream,
@@ -1754,23 +1818,54 @@ int pl_sv_reset(pl_sv_t h) { return pl_sv_set_basis_state(h, 0); }
 int pl_sv_device_ptr(pl_sv_t h, void **out) {
     return guarded([&] {
         if (!h || !out) throw InvalidArg("null argument");
+        flush_deferred(h); // the caller may read or write the memory itself
         *out = h->d;
     });
 }
 int pl_sv_synchronize(pl_sv_t h) {
     return guarded([&] {
         if (!h) throw InvalidArg("null argument");
+        flush_deferred(h);
         HIP_OK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int pl_sv_set_deferred(pl_sv_t h, int on) {
+    return guarded([&] {
+        if (!h) throw InvalidArg("null argument");
+        if (!on) {
+            flush_deferred(h);
+        }
+        h->defer = on != 0;
+    });
+}
+int pl_sv_deferred_count(pl_sv_t h, int *out) {
+    return guarded([&] {
+        if (!h || !out) throw InvalidArg("null argument");
+        *out = static_cast<int>(h->q_ids.size());
     });
 }
 
 int pl_sv_apply_gate(pl_sv_t h, int id, const int *wires, int nw, int inverse,
                      const double *params, int np) {
+    if (h && h->defer) {
+        return guarded([&] { defer_gate(h, id, wires, nw, inverse != 0, params, np); });
+    }
     return dispatch(h, [&](auto &E) { E.gate(E.psi(), id, wires, nw, inverse != 0, params, np); });
 }
-
 int pl_sv_apply_ops(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wires,
                     const uint8_t *inverse, const int *nparams, const double *params) {
+    if (h && h->defer) {
+        return guarded([&] {
+            for (int i = 0, wo = 0, po = 0; i < nops; i++) {
+                if (ids[i] != -1) {
+                    defer_gate(h, ids[i], wires + wo, nwires[i], inverse[i] != 0, params + po, nparams[i]);
+                }
+                wo += nwires[i];
+                po += nparams[i];
+            }
+        });
+    }
     return dispatch(h, [&](auto &E) {
         for (int i = 0, wo = 0, po = 0; i < nops; i++) {
             if (ids[i] != -1) {
@@ -1905,6 +2000,8 @@ int pl_sv_distance(pl_sv_t a, pl_sv_t b, double *out) {
             throw InvalidArg("states must have the same number of qubits");
         }
         HIP_OK(hipSetDevice(0));
+        flush_deferred(a);
+        flush_deferred(b);
         HIP_OK(hipStreamSynchronize(b->stream)); // b's pending work, then run on a's stream
         const unsigned grid = std::min<unsigned>(grid_for(a->len, 1, 4), kReduceBlocks);
         auto *part = reinterpret_cast<double2 *>(reduce_buf(a, (kReduceBlocks + 1) * sizeof(double2)));
@@ -2199,12 +2296,14 @@ int pl_generate_mcmc_samples(pl_sv_t h, int kernel, int64_t num_burnin, int64_t 
 int pl_timer_start(pl_sv_t h) {
     return guarded([&] {
         if (!h) throw InvalidArg("null argument");
+        flush_deferred(h);
         HIP_OK(hipEventRecord(h->ev0, h->stream));
     });
 }
 int pl_timer_stop(pl_sv_t h, double *ms) {
     return guarded([&] {
         if (!h || !ms) throw InvalidArg("null argument");
+        flush_deferred(h);
         HIP_OK(hipEventRecord(h->ev1, h->stream));
         HIP_OK(hipEventSynchronize(h->ev1));
         float f = 0;
@@ -2223,6 +2322,7 @@ int pl_event_record(pl_sv_t h, int slot) {
         if (!h->slots[slot]) {
             HIP_OK(hipEventCreate(&h->slots[slot]));
         }
+        flush_deferred(h);
         HIP_OK(hipEventRecord(h->slots[slot], h->stream));
     });
 }
