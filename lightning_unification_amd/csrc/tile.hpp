@@ -72,7 +72,7 @@ class TileScheduler {
   public:
     TileScheduler(int n, const TapeView &t, int k, int low)
         : n_((tile_check_args(n, k, low), n)), low_(low), budget_(k - low), low_mask_((uint64_t{1} << low) - 1),
-          all_(n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1), deep_(n >= kTileDeepSearchQubits),
+          all_(0), deep_(n >= kTileDeepSearchQubits),
           bits_(t.nops, 0), ok_(t.nops, 0) {
         std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
         for (int i = 0; i < t.nops; i++) {
@@ -105,6 +105,12 @@ class TileScheduler {
             return false;
         }
         p = TilePass{};
+        // every bit the scanned operations touch: once all of them are
+        // blocked, no later operation can join
+        all_ = 0;
+        for (size_t j = 0; j < pending_.size() && j < kTileLookahead; j++) {
+            all_ |= bits_[pending_[j]];
+        }
         // first guess: high bits claimed in tape order until the budget is spent
         uint64_t hmask = 0, blocked = 0;
         for (size_t j = 0; j < pending_.size() && j < kTileLookahead && blocked != all_; j++) {
@@ -193,7 +199,7 @@ class TileScheduler {
                 if (left) {
                     left->push_back(i);
                 }
-                if (blocked == all_) {
+                if ((blocked & all_) == all_) {
                     j++;
                     break;
                 }
@@ -233,7 +239,7 @@ class TileScheduler {
     }
 
     int n_, low_, budget_;
-    uint64_t low_mask_, all_;
+    uint64_t low_mask_, all_; // all_: the bits of the operations the current pass may take
     bool deep_;
     std::vector<uint64_t> bits_;
     std::vector<char> ok_;
