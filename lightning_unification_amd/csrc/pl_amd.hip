@@ -466,27 +466,48 @@ template <class T> struct Engine {
             throw InvalidArg("a tile of 2^" + std::to_string(k) + " amplitudes exceeds the LDS of a workgroup");
         }
         // the passes' programs, uploaded pass by pass into one device buffer
-        // (instructions, then the LDS-applied ops): a program holds at most one
-        // instruction and one LDS op per operation, so the tape's length bounds
-        // both, and the host copies never move while a copy may read them
-        const size_t cap = static_cast<size_t>(std::max(t.nops, 1));
+        // (instructions, then the LDS-applied ops).  A pass's program holds at
+        // most one instruction and one LDS op per operation.  The buffers hold
+        // `cap` of each; when the next pass could overflow them, the passes
+        // queued so far are waited for and the buffers start over (grown if
+        // that one pass needs more), so host copies never move while a copy
+        // may read them and no queued pass reads a slot being rewritten.
         std::vector<TrInst<T>> prog;
         std::vector<TpOp<T>> lops;
-        prog.reserve(cap);
-        lops.reserve(cap);
-        const size_t pbytes = (cap * sizeof(TrInst<T>) + 255) & ~size_t{255};
-        unsigned char *dev = nullptr;
-        HIP_OK(hipMalloc(&dev, pbytes + cap * sizeof(TpOp<T>)));
+        size_t cap = 0, pbytes = 0;
         struct Free {
-            void *p;
+            void *p = nullptr;
             hipStream_t s;
             ~Free() {
-                (void)hipStreamSynchronize(s);
-                (void)hipFree(p);
+                if (p) {
+                    (void)hipStreamSynchronize(s);
+                    (void)hipFree(p);
+                }
             }
-        } guard{dev, h->stream};
-        auto *dprog = reinterpret_cast<TrInst<T> *>(dev);
-        auto *dlops = reinterpret_cast<TpOp<T> *>(dev + pbytes);
+        } guard{nullptr, h->stream};
+        TrInst<T> *dprog = nullptr;
+        TpOp<T> *dlops = nullptr;
+        auto room_for = [&](size_t nops_pass) {
+            if (prog.size() + nops_pass <= cap && lops.size() + nops_pass <= cap) {
+                return;
+            }
+            HIP_OK(hipStreamSynchronize(h->stream));
+            prog.clear();
+            lops.clear();
+            if (nops_pass > cap) {
+                if (guard.p) {
+                    HIP_OK(hipFree(guard.p));
+                    guard.p = nullptr;
+                }
+                cap = std::max<size_t>(nops_pass, std::min<size_t>(static_cast<size_t>(t.nops), kTileLookahead));
+                prog.reserve(cap);
+                lops.reserve(cap);
+                pbytes = (cap * sizeof(TrInst<T>) + 255) & ~size_t{255};
+                HIP_OK(hipMalloc(&guard.p, pbytes + cap * sizeof(TpOp<T>)));
+                dprog = reinterpret_cast<TrInst<T> *>(guard.p);
+                dlops = reinterpret_cast<TpOp<T> *>(static_cast<unsigned char *>(guard.p) + pbytes);
+            }
+        };
         const uint64_t ntiles = uint64_t{1} << (n() - k);
         int npasses = 0;
         TilePass ps;
@@ -499,10 +520,11 @@ template <class T> struct Engine {
                 continue;
             }
             ps.hb = tile_lane_order<T>(n(), t, ps, low, woff, poff); // lane bits where its ops are cheapest
+            room_for(ps.ops.size());
             const size_t p0 = prog.size(), l0 = lops.size();
             tile_program<T>(n(), t, ps, low, woff, poff, prog, lops);
             if (prog.size() > cap || lops.size() > cap) {
-                throw std::logic_error("tile programs longer than their tape");
+                throw std::logic_error("a tile program longer than its pass");
             }
             const int ni = static_cast<int>(prog.size() - p0);
             if (ni == 0) {

@@ -83,3 +83,20 @@ def test_tiled_defaults(oracle, dtype, n):
         sv.apply_tape_tiled(tape)
         got = sv.state()
     assert rel_err(got, exp) < TOL[dtype]
+
+
+def test_tiled_long_tape_refills_program_buffers(oracle):
+    """9000 gates in one call: the executor's program buffers hold 4096
+    operations' worth, so it waits for the queued passes and starts them
+    over (more than once) part-way through the tape."""
+    from lightning_unification_amd import capi
+    n = 12
+    tape = zoo_tape(n, 77, count=8900)
+    psi = random_state(n, 31)
+    exp = oracle.apply_tape(psi.copy(), without_prep(tape))
+    print(f"tiled long tape: {len(tape.ops)} ops", flush=True)
+    with capi.DeviceStateVector.from_array(psi) as sv:
+        npass = sv.apply_tape_tiled(tape, 11, 3)
+        got = sv.state()
+    assert npass == len(capi.tile_plan(n, tape, 11, 3))
+    assert rel_err(got, exp) < 1e-12
