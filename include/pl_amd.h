@@ -114,7 +114,8 @@ int pl_sv_apply_ops(pl_sv_t sv, int nops, const int *ids, const int *nwires,
  * the state (measurements, adjoint, copies, downloads, pl_sv_device_ptr,
  * pl_sv_synchronize, timing events), and whenever it holds 4096 gates.  A
  * call that overwrites the whole state (pl_sv_upload, pl_sv_set_basis_state,
- * pl_sv_reset, pl_sv_copy into sv) drops it.  on = 0 applies the queue and
+ * pl_sv_reset, pl_sv_copy into sv) drops it; pl_expval_noalloc, which must
+ * not allocate, fails while gates are queued.  on = 0 applies the queue and
  * returns to gate-by-gate application.  Off by default. */
 int pl_sv_set_deferred(pl_sv_t sv, int on);
 /* Gates queued on sv and not yet applied. */
@@ -149,8 +150,10 @@ int pl_var(pl_sv_t sv, const int *prog, const double *coeffs,
  * PauliZ / Identity factors) and fails with PL_ERR_INVALID otherwise.  Its
  * term tables and partial sums live in a fixed 2 MiB device array per GPU
  * (held until the process ends); an observable needing more fails with
- * PL_ERR_INVALID (a 100-term Hamiltonian needs about 0.2 MiB).  Calls that
- * overlap on one GPU share that array and race. */
+ * PL_ERR_INVALID (a 100-term Hamiltonian needs about 0.2 MiB), and so does a
+ * call while deferred gates are queued on sv (applying them may allocate;
+ * pl_sv_synchronize applies them first).  Calls that overlap on one GPU
+ * share that array and race. */
 int pl_expval_noalloc(pl_sv_t sv, const int *prog, const double *coeffs,
                       const double *mats, double *out);
 /* Measurements::expval/var(name, wires) (MeasurementsLQubit.hpp:167-181,298-317). */

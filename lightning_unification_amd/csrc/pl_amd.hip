@@ -2112,6 +2112,12 @@ unsigned char *noalloc_scratch() {
 } // namespace
 
 int pl_expval_noalloc(pl_sv_t h, const int *prog, const double *coeffs, const double *mats, double *out) {
+    if (h && !h->q_ids.empty()) { // applying queued gates may allocate
+        return guarded([] {
+            throw InvalidArg("gates are queued on this state (pl_sv_set_deferred): call pl_sv_synchronize "
+                             "before the allocation-free expectation value");
+        });
+    }
     return dispatch(h, [&](auto &E) {
         if (!prog || !out) {
             throw InvalidArg("null argument");

@@ -64,13 +64,12 @@ def _use_results(sv, n):
         "event_record": lambda: (sv.event_record(0), sv.state())[1],
         "apply_matrix": lambda: (sv.apply_matrix(np.eye(2) * 1j, [2]), sv.state())[1],
         "apply_generator": lambda: (sv.apply_generator("RX", [1]), sv.state())[1],
-        "expval_noalloc": lambda: sv.expval_noalloc(ham),
     }
 
 
 @pytest.mark.parametrize("use", ["norm2", "expval", "var", "expval_named", "probs", "sample",
                                  "synchronize", "device_ptr", "event_record", "apply_matrix",
-                                 "apply_generator", "expval_noalloc"])
+                                 "apply_generator"])
 def test_each_use_applies_the_queue_first(use):
     n = 11
     tape = random_layers(n, 4, seed=21)
@@ -191,3 +190,19 @@ def test_turning_deferral_off_applies_the_queue(oracle):
         t = random_layers(n, 2, seed=6)
         t.add("Hadamard", [3])
         assert rel_err(sv.state(), oracle.apply_tape(psi.copy(), t)) < 1e-12
+
+
+def test_allocation_free_expval_refuses_a_queue():
+    """pl_expval_noalloc must not allocate, and applying queued gates may:
+    with gates queued it fails (ValueError) and leaves them queued; after
+    pl_sv_synchronize it equals pl_expval."""
+    n = 11
+    ham = random_hamiltonian(n, 12, seed=9)
+    tape = random_layers(n, 3, seed=2)
+    _, eager, lazy = _pair(n, np.complex128, tape)
+    with eager, lazy:
+        with pytest.raises(ValueError):
+            lazy.expval_noalloc(ham)
+        assert lazy.deferred_count() == len(tape.ops)
+        lazy.synchronize()
+        assert rel_err(lazy.expval_noalloc(ham), eager.expval(ham)) < 1e-12
