@@ -384,11 +384,15 @@ __device__ __forceinline__ double pauli_flip(double v, uint32_t par) {
 // (scalar loads).
 // HIGH: high bits of a full tile, 2^HIGH threads per workgroup.  FULL: every
 // tile holds 2^(3 + HIGH) amplitudes (no element or pair is out of range).
-template <class T, int HIGH, bool FULL>
+// SIGNS: the pass's terms are terms[t0 .. t0 + nt) with nt <= 64; lane l
+// keeps term t0 + l's bits outside S, and one ballot per tile gives every
+// term's sign there as a bit of a scalar mask (else each term computes its
+// own with a scalar popcount).
+template <class T, int HIGH, bool FULL, bool SIGNS>
 __global__ __launch_bounds__(1 << HIGH) void k_pauli_tile(const v2<T> *__restrict__ sv, uint64_t ntiles,
                                                        const PauliTileGeom geo, int ngroups,
                                                        const PauliTileGroup *__restrict__ groups,
-                                                       const PauliTileTerm *__restrict__ terms,
+                                                       const PauliTileTerm *__restrict__ terms, int t0, int nt,
                                                        double2 *__restrict__ partial) {
     extern __shared__ unsigned char smem_raw[];
     v2<T> *tile = reinterpret_cast<v2<T> *>(smem_raw);
@@ -417,6 +421,13 @@ __global__ __launch_bounds__(1 << HIGH) void k_pauli_tile(const v2<T> *__restric
         }
         return base;
     };
+    uint64_t zyo_lane = 0; // SIGNS: this lane's term's bits outside S
+    if constexpr (SIGNS) {
+        const int l = static_cast<int>(tid & 63U);
+        if (l < nt) {
+            zyo_lane = terms[t0 + l].zyo;
+        }
+    }
     v2<T> nxt[kPauliPerThread];
     uint64_t q = blockIdx.x;
     uint64_t base = tile_base(q);
@@ -449,11 +460,18 @@ __global__ __launch_bounds__(1 << HIGH) void k_pauli_tile(const v2<T> *__restric
                 }
             }
         }
+        // SIGNS: bit l = parity of term t0 + l outside S on this tile
+        uint64_t smask = 0;
+        if constexpr (SIGNS) {
+            smask = __ballot(__popcll(cur & zyo_lane) & 1);
+        }
         // one bucket's terms against the transform value (wr, wi)
-        auto bucket = [&](int t0, int t1, uint32_t e0, double wr, double wi) {
-            for (int t = t0; t < t1; t++) {
+        auto bucket = [&](int tb0, int tb1, uint32_t e0, double wr, double wi) {
+            for (int t = tb0; t < tb1; t++) {
                 const PauliTileTerm tm = terms[t];
-                const uint32_t par = __popc(e0 & tm.zyl) + __popcll(cur & tm.zyo);
+                const uint32_t po = SIGNS ? static_cast<uint32_t>((smask >> (t - t0)) & 1U)
+                                          : static_cast<uint32_t>(__popcll(cur & tm.zyo));
+                const uint32_t par = __popc(e0 & tm.zyl) + po;
                 acc += pauli_flip(fma(tm.ci, wi, tm.cr * wr), par);
             }
         };

@@ -658,6 +658,7 @@ template <class T> struct Engine {
         // per pass: geometry, groups and compressed terms, all staged together
         std::vector<PauliTileGeom> geos;
         std::vector<std::pair<size_t, size_t>> gspan; // offsets into the group table
+        std::vector<std::pair<size_t, size_t>> tspan; // the pass's terms: first, count
         std::vector<PauliTileGroup> gtab;
         std::vector<PauliTileTerm> ttab;
         for (size_t pi = 0; pi < passes.size(); pi++) {
@@ -683,7 +684,7 @@ template <class T> struct Engine {
                 return r;
             };
             geos.push_back(g);
-            const size_t g0 = gtab.size();
+            const size_t g0 = gtab.size(), t_first = ttab.size();
             for (const Item *it : ps.its) {
                 const uint64_t x = it->x;
                 PauliTileGroup gr{};
@@ -713,6 +714,7 @@ template <class T> struct Engine {
                 gtab.push_back(gr);
             }
             gspan.emplace_back(g0, gtab.size() - g0);
+            tspan.emplace_back(t_first, ttab.size() - t_first);
         }
         std::vector<PauliTerm> wtab;
         std::vector<size_t> wstart;
@@ -762,17 +764,23 @@ template <class T> struct Engine {
                 const unsigned grid = static_cast<unsigned>(
                     std::min<uint64_t>({ntiles, resident, static_cast<uint64_t>(kReduceBlocks)}));
                 hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), lds, h->stream, sv, ntiles, g, ng,
-                                   dgroups + gspan[pi].first, dtterms, part + total);
+                                   dgroups + gspan[pi].first, dtterms, static_cast<int>(tspan[pi].first),
+                                   static_cast<int>(tspan[pi].second), part + total);
                 return grid;
             };
-            unsigned grid = 0;
-            if (hmax == kPauliTileHighMax) {
-                grid = full ? launch(k_pauli_tile<T, kPauliTileHighMax, true>, 1 << kPauliTileHighMax)
-                            : launch(k_pauli_tile<T, kPauliTileHighMax, false>, 1 << kPauliTileHighMax);
-            } else {
-                grid = full ? launch(k_pauli_tile<T, kPauliTileHigh, true>, 1 << kPauliTileHigh)
-                            : launch(k_pauli_tile<T, kPauliTileHigh, false>, 1 << kPauliTileHigh);
-            }
+            // a pass of at most 64 terms takes its out-of-tile signs from one ballot per tile
+            const bool signs = tspan[pi].second <= 64;
+            auto pick = [&](auto high) -> unsigned {
+                constexpr int H = decltype(high)::value;
+                if (full) {
+                    return signs ? launch(k_pauli_tile<T, H, true, true>, 1 << H)
+                                 : launch(k_pauli_tile<T, H, true, false>, 1 << H);
+                }
+                return signs ? launch(k_pauli_tile<T, H, false, true>, 1 << H)
+                             : launch(k_pauli_tile<T, H, false, false>, 1 << H);
+            };
+            const unsigned grid = hmax == kPauliTileHighMax ? pick(std::integral_constant<int, kPauliTileHighMax>{})
+                                                            : pick(std::integral_constant<int, kPauliTileHigh>{});
             HIP_OK(hipGetLastError());
             total += grid;
         }
