@@ -1,10 +1,12 @@
 // Gate fusion for the precision path: operations of a tape whose wires
 // together span at most `max_wires` wires are multiplied into one dense block
-// (2^k x 2^k, k <= 5).  A block is grown from the first operation not yet
-// placed by scanning ahead in tape order: an operation joins when its bits
-// fit the block and it shares no bit with any operation the scan has passed
+// (2^k x 2^k, k <= 5).  A block has a wire set; scanning ahead in tape order
+// from the first operation not yet placed, an operation joins when its bits
+// lie in the set and it shares no bit with any operation the scan has passed
 // over, so it only moves ahead of operations on disjoint bits, with which it
-// commutes (the rule of the tile-pass scheduler).  Each block's matrix is
+// commutes (the rule of the tile-pass scheduler).  The set starts as the
+// wires claimed in tape order and is improved by exchanging wires while the
+// block takes more operations.  Each block's matrix is
 // built on the host by
 // applying the very group ops the GPU kernels apply (lower_gate) to the 2^k
 // basis vectors of a k-qubit local system, so fused and unfused runs share
@@ -74,7 +76,9 @@ struct TapeView {
     const double *params;
 };
 
-constexpr int kFuseLookahead = 512; // operations scanned ahead per block
+constexpr int kFuseLookahead = 512;  // operations scanned ahead per block
+constexpr int kFuseSearchRounds = 16; // wire-set exchanges tried per block, at most
+constexpr int kFuseDeepQubits = 28;   // states from which two-wire exchanges are tried too
 
 inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires, int pad_to) {
     if (max_wires < 1 || max_wires > 5) {
@@ -150,30 +154,127 @@ inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires
         if (pos >= t.nops) {
             break;
         }
-        placed[pos] = 1;
         FusedBlock b;
-        b.ops.push_back(pos);
         if (t.nwires[pos] > max_wires) { // applied unfused
+            placed[pos] = 1;
+            b.ops.push_back(pos);
             b.fused = false;
             out.push_back(std::move(b));
             continue;
         }
-        uint64_t cur = mask[pos], blocked = 0;
-        const uint64_t all = n >= 64 ? ~uint64_t{0} : (uint64_t{1} << n) - 1;
-        for (int j = pos + 1; j < t.nops && j <= pos + kFuseLookahead && blocked != all; j++) {
-            if (placed[j] || t.ids[j] == -1) {
-                continue;
-            }
-            const uint64_t u = cur | mask[j];
-            if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && __builtin_popcountll(u) <= max_wires) {
-                cur = u;
-                placed[j] = 1;
-                b.ops.push_back(j);
-            } else {
-                blocked |= mask[j];
+        const int end = std::min(t.nops, pos + kFuseLookahead + 1);
+        uint64_t all = 0; // every bit the scanned operations touch
+        for (int j = pos; j < end; j++) {
+            if (!placed[j] && t.ids[j] != -1) {
+                all |= mask[j];
             }
         }
-        build(b, cur);
+        // the operations a block on wire set `set` takes, in order: each one
+        // inside the set that shares no bit with an operation passed over
+        auto absorb = [&](uint64_t set, std::vector<int> *take) {
+            uint64_t blocked = 0;
+            int taken = 0;
+            for (int j = pos; j < end && (blocked & all) != all; j++) {
+                if (placed[j] || t.ids[j] == -1) {
+                    continue;
+                }
+                if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && (mask[j] & ~set) == 0) {
+                    taken++;
+                    if (take) {
+                        take->push_back(j);
+                    }
+                } else {
+                    blocked |= mask[j];
+                }
+            }
+            return taken;
+        };
+        // first guess: wires claimed in tape order from the first unplaced
+        // operation on
+        uint64_t set = mask[pos];
+        {
+            uint64_t blocked = 0;
+            for (int j = pos + 1; j < end && (blocked & all) != all; j++) {
+                if (placed[j] || t.ids[j] == -1) {
+                    continue;
+                }
+                const uint64_t u = set | mask[j];
+                if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && __builtin_popcountll(u) <= max_wires) {
+                    set = u;
+                } else {
+                    blocked |= mask[j];
+                }
+            }
+        }
+        // then add or exchange wires while the block takes more operations
+        // (the rule of the tile-pass scheduler; the first operation need not
+        // stay: one left out opens a later block).  On the 50-layer random
+        // circuit of 33 qubits: 185 blocks, 163 with single exchanges, 148
+        // with pairs.
+        int best = absorb(set, nullptr);
+        for (int round = 0; round < kFuseSearchRounds; round++) {
+            uint64_t best_set = set;
+            int best_n = best;
+            for (int b2 = 0; b2 < n; b2++) {
+                if ((set >> b2) & 1U) {
+                    continue;
+                }
+                const uint64_t in = uint64_t{1} << b2;
+                if (__builtin_popcountll(set) < max_wires) {
+                    const int c = absorb(set | in, nullptr);
+                    if (c > best_n) {
+                        best_n = c;
+                        best_set = set | in;
+                    }
+                }
+                for (int a = 0; a < n; a++) {
+                    if ((set >> a) & 1U) {
+                        const uint64_t m = (set & ~(uint64_t{1} << a)) | in;
+                        const int c = absorb(m, nullptr);
+                        if (c > best_n) {
+                            best_n = c;
+                            best_set = m;
+                        }
+                    }
+                }
+            }
+            if (best_n == best && n >= kFuseDeepQubits) { // the first two-wire exchange that helps
+                for (int a = 0; a < n && best_n == best; a++) {
+                    for (int a2 = a + 1; a2 < n && best_n == best && ((set >> a) & 1U); a2++) {
+                        if (!((set >> a2) & 1U)) {
+                            continue;
+                        }
+                        for (int c1 = 0; c1 < n && best_n == best; c1++) {
+                            for (int c2 = c1 + 1; c2 < n && !((set >> c1) & 1U); c2++) {
+                                if ((set >> c2) & 1U) {
+                                    continue;
+                                }
+                                const uint64_t m = (set & ~(uint64_t{1} << a) & ~(uint64_t{1} << a2)) |
+                                                   (uint64_t{1} << c1) | (uint64_t{1} << c2);
+                                const int c = absorb(m, nullptr);
+                                if (c > best) {
+                                    best_n = c;
+                                    best_set = m;
+                                    break;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (best_n == best) {
+                break;
+            }
+            best = best_n;
+            set = best_set;
+        }
+        absorb(set, &b.ops);
+        uint64_t used = 0; // the block's width: the wires its operations touch
+        for (int j : b.ops) {
+            placed[j] = 1;
+            used |= mask[j];
+        }
+        build(b, used);
         out.push_back(std::move(b));
     }
     return out;
