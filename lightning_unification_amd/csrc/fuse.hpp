@@ -80,93 +80,58 @@ constexpr int kFuseLookahead = 512;  // operations scanned ahead per block
 constexpr int kFuseSearchRounds = 16; // wire-set exchanges tried per block, at most
 constexpr int kFuseDeepQubits = 28;   // states from which two-wire exchanges are tried too
 
-inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires, int pad_to) {
-    if (max_wires < 1 || max_wires > 5) {
-        throw InvalidArg("fusion width must be in [1, 5]");
-    }
-    pad_to = std::min(pad_to, n);
-    std::vector<int> woff(t.nops + 1, 0), poff(t.nops + 1, 0);
-    std::vector<uint64_t> mask(t.nops, 0);
-    for (int i = 0; i < t.nops; i++) {
-        if (t.nwires[i] < 0 || t.nparams[i] < 0) {
-            throw InvalidArg("negative wire or parameter count");
+// The blocks of a tape, one at a time (the executor builds and launches a
+// block while it plans the next).
+class FuseScheduler {
+  public:
+    FuseScheduler(int n, const TapeView &t, int max_wires, int pad_to)
+        : n_(n), t_(t), max_wires_(max_wires), pad_to_(std::min(pad_to, n)), woff_(t.nops + 1, 0),
+          poff_(t.nops + 1, 0), mask_(t.nops, 0), placed_(t.nops, 0) {
+        if (max_wires < 1 || max_wires > 5) {
+            throw InvalidArg("fusion width must be in [1, 5]");
         }
-        woff[i + 1] = woff[i] + t.nwires[i];
-        poff[i + 1] = poff[i] + t.nparams[i];
-        if (t.ids[i] == -1) {
-            continue;
-        }
-        for (int a = 0; a < t.nwires[i]; a++) {
-            const int w = t.wires[woff[i] + a];
-            if (w < 0 || w >= n) {
-                throw InvalidArg("wire index out of range");
+        for (int i = 0; i < t.nops; i++) {
+            if (t.nwires[i] < 0 || t.nparams[i] < 0) {
+                throw InvalidArg("negative wire or parameter count");
             }
-            mask[i] |= uint64_t{1} << (n - 1 - w);
-        }
-    }
-    std::vector<FusedBlock> out;
-
-    auto build = [&](FusedBlock &b, uint64_t bitmask) {
-        for (int p = 0; p < n; p++) {
-            if ((bitmask >> p) & 1U) {
-                b.bits.push_back(p);
+            woff_[i + 1] = woff_[i] + t.nwires[i];
+            poff_[i + 1] = poff_[i] + t.nparams[i];
+            if (t.ids[i] == -1) {
+                continue;
             }
-        }
-        for (int p = 0; static_cast<int>(b.bits.size()) < pad_to && p < n; p++) {
-            if (!((bitmask >> p) & 1U)) {
-                b.bits.push_back(p);
-            }
-        }
-        std::sort(b.bits.begin(), b.bits.end());
-        const int k = static_cast<int>(b.bits.size());
-        const size_t dim = size_t{1} << k;
-        std::vector<GroupOp<double>> ops;
-        for (int i : b.ops) {
-            std::vector<int> lw(t.nwires[i]);
             for (int a = 0; a < t.nwires[i]; a++) {
-                const int bit = n - 1 - t.wires[woff[i] + a];
-                const int m = static_cast<int>(std::find(b.bits.begin(), b.bits.end(), bit) - b.bits.begin());
-                lw[a] = k - 1 - m; // local wire of local bit m
-            }
-            ops.push_back(lower_gate<double>(k, t.ids[i], lw.data(), t.nwires[i], t.inverse[i] != 0,
-                                             t.params + poff[i], t.nparams[i]));
-        }
-        b.U.assign(dim * dim, 0);
-        std::vector<std::complex<double>> col(dim);
-        for (size_t c = 0; c < dim; c++) {
-            std::fill(col.begin(), col.end(), 0);
-            col[c] = 1;
-            for (const auto &g : ops) {
-                host_apply_group(g, col.data(), k);
-            }
-            for (size_t r = 0; r < dim; r++) {
-                b.U[r * dim + c] = col[r];
+                const int w = t.wires[woff_[i] + a];
+                if (w < 0 || w >= n) {
+                    throw InvalidArg("wire index out of range");
+                }
+                mask_[i] |= uint64_t{1} << (n - 1 - w);
             }
         }
-    };
+    }
 
-    std::vector<char> placed(t.nops, 0);
-    int pos = 0;
-    while (true) {
-        while (pos < t.nops && (placed[pos] || t.ids[pos] == -1)) { // state preparation: skipped
-            pos++;
+    // The next block, its matrix built (false once every operation is placed).
+    bool next(FusedBlock &b) {
+        const TapeView &t = t_;
+        const int n = n_, max_wires = max_wires_;
+        while (pos_ < t.nops && (placed_[pos_] || t.ids[pos_] == -1)) { // state preparation: skipped
+            pos_++;
         }
-        if (pos >= t.nops) {
-            break;
+        if (pos_ >= t.nops) {
+            return false;
         }
-        FusedBlock b;
+        const int pos = pos_;
+        b = FusedBlock{};
         if (t.nwires[pos] > max_wires) { // applied unfused
-            placed[pos] = 1;
+            placed_[pos] = 1;
             b.ops.push_back(pos);
             b.fused = false;
-            out.push_back(std::move(b));
-            continue;
+            return true;
         }
         const int end = std::min(t.nops, pos + kFuseLookahead + 1);
         uint64_t all = 0; // every bit the scanned operations touch
         for (int j = pos; j < end; j++) {
-            if (!placed[j] && t.ids[j] != -1) {
-                all |= mask[j];
+            if (!placed_[j] && t.ids[j] != -1) {
+                all |= mask_[j];
             }
         }
         // the operations a block on wire set `set` takes, in order: each one
@@ -175,34 +140,34 @@ inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires
             uint64_t blocked = 0;
             int taken = 0;
             for (int j = pos; j < end && (blocked & all) != all; j++) {
-                if (placed[j] || t.ids[j] == -1) {
+                if (placed_[j] || t.ids[j] == -1) {
                     continue;
                 }
-                if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && (mask[j] & ~set) == 0) {
+                if (t.nwires[j] <= max_wires && !(mask_[j] & blocked) && (mask_[j] & ~set) == 0) {
                     taken++;
                     if (take) {
                         take->push_back(j);
                     }
                 } else {
-                    blocked |= mask[j];
+                    blocked |= mask_[j];
                 }
             }
             return taken;
         };
         // first guess: wires claimed in tape order from the first unplaced
         // operation on
-        uint64_t set = mask[pos];
+        uint64_t set = mask_[pos];
         {
             uint64_t blocked = 0;
             for (int j = pos + 1; j < end && (blocked & all) != all; j++) {
-                if (placed[j] || t.ids[j] == -1) {
+                if (placed_[j] || t.ids[j] == -1) {
                     continue;
                 }
-                const uint64_t u = set | mask[j];
-                if (t.nwires[j] <= max_wires && !(mask[j] & blocked) && __builtin_popcountll(u) <= max_wires) {
+                const uint64_t u = set | mask_[j];
+                if (t.nwires[j] <= max_wires && !(mask_[j] & blocked) && __builtin_popcountll(u) <= max_wires) {
                     set = u;
                 } else {
-                    blocked |= mask[j];
+                    blocked |= mask_[j];
                 }
             }
         }
@@ -271,10 +236,69 @@ inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires
         absorb(set, &b.ops);
         uint64_t used = 0; // the block's width: the wires its operations touch
         for (int j : b.ops) {
-            placed[j] = 1;
-            used |= mask[j];
+            placed_[j] = 1;
+            used |= mask_[j];
         }
         build(b, used);
+        return true;
+    }
+
+  private:
+    void build(FusedBlock &b, uint64_t bitmask) const {
+        const TapeView &t = t_;
+        const int n = n_;
+        for (int p = 0; p < n; p++) {
+            if ((bitmask >> p) & 1U) {
+                b.bits.push_back(p);
+            }
+        }
+        for (int p = 0; static_cast<int>(b.bits.size()) < pad_to_ && p < n; p++) {
+            if (!((bitmask >> p) & 1U)) {
+                b.bits.push_back(p);
+            }
+        }
+        std::sort(b.bits.begin(), b.bits.end());
+        const int k = static_cast<int>(b.bits.size());
+        const size_t dim = size_t{1} << k;
+        std::vector<GroupOp<double>> ops;
+        for (int i : b.ops) {
+            std::vector<int> lw(t.nwires[i]);
+            for (int a = 0; a < t.nwires[i]; a++) {
+                const int bit = n - 1 - t.wires[woff_[i] + a];
+                const int m = static_cast<int>(std::find(b.bits.begin(), b.bits.end(), bit) - b.bits.begin());
+                lw[a] = k - 1 - m; // local wire of local bit m
+            }
+            ops.push_back(lower_gate<double>(k, t.ids[i], lw.data(), t.nwires[i], t.inverse[i] != 0,
+                                             t.params + poff_[i], t.nparams[i]));
+        }
+        b.U.assign(dim * dim, 0);
+        std::vector<std::complex<double>> col(dim);
+        for (size_t c = 0; c < dim; c++) {
+            std::fill(col.begin(), col.end(), 0);
+            col[c] = 1;
+            for (const auto &g : ops) {
+                host_apply_group(g, col.data(), k);
+            }
+            for (size_t r = 0; r < dim; r++) {
+                b.U[r * dim + c] = col[r];
+            }
+        }
+    }
+
+    int n_;
+    TapeView t_;
+    int max_wires_, pad_to_;
+    std::vector<int> woff_, poff_;
+    std::vector<uint64_t> mask_;
+    std::vector<char> placed_;
+    int pos_ = 0;
+};
+
+inline std::vector<FusedBlock> fuse_tape(int n, const TapeView &t, int max_wires, int pad_to) {
+    FuseScheduler s(n, t, max_wires, pad_to);
+    std::vector<FusedBlock> out;
+    FusedBlock b;
+    while (s.next(b)) {
         out.push_back(std::move(b));
     }
     return out;

@@ -277,53 +277,25 @@ template <class T> struct Engine {
         return g;
     }
 
-    // Applies a tape as fused blocks; returns the number of blocks.
+    // Applies a tape as fused blocks; returns the number of blocks.  Each
+    // block is planned, built, uploaded and launched before the next is
+    // planned, so the host's fusion overlaps the device's blocks.
     int apply_fused(V *sv, const TapeView &t, int max_wires, int flags, std::vector<FusedBlock> *keep) {
         const bool probe_mfma = kMfma && !(flags & PL_FUSE_NO_MFMA) && n() >= 5 + kF5ColBits;
-        auto blocks = fuse_tape(n(), t, max_wires, probe_mfma ? 5 : 0);
+        FuseScheduler sched(n(), t, max_wires, probe_mfma ? 5 : 0);
         const bool renorm = flags & PL_FUSE_RENORM;
 
-        // one device buffer for every block matrix (no staging reuse, no sync per block)
-        std::vector<size_t> moff(blocks.size(), 0);
-        size_t mbytes = 0;
-        for (size_t i = 0; i < blocks.size(); i++) {
-            const auto &b = blocks[i];
-            moff[i] = mbytes;
-            if (!b.fused) {
-                continue;
-            }
-            if (mfma_ok(b, flags)) {
-                mbytes += sizeof(Fused5Mat);
-            } else if (b.U.size() >= 64) {
-                mbytes += b.U.size() * sizeof(dcx<T>);
-            }
-            mbytes = (mbytes + 255) & ~size_t{255};
-        }
+        // block matrices go through kFuseSlots slots of host and device memory;
+        // when they run out, the blocks queued so far are waited for and the
+        // slots start over, so a host slot is never rewritten while its copy
+        // may be pending and a device slot never while its block may run
+        constexpr size_t kFuseSlots = 256;
+        const size_t slot = (std::max(sizeof(Fused5Mat), size_t{32 * 32} * sizeof(dcx<T>)) + 255) & ~size_t{255};
         const uint64_t ntiles = n() >= 5 + kF5ColBits ? (len() >> (5 + kF5ColBits)) : 1;
-        const size_t tail = ntiles * sizeof(double) + 256;
-        std::vector<unsigned char> host(mbytes + tail, 0);
-        for (size_t i = 0; i < blocks.size(); i++) {
-            const auto &b = blocks[i];
-            if (!b.fused) {
-                continue;
-            }
-            if (mfma_ok(b, flags)) {
-                auto *fm = reinterpret_cast<Fused5Mat *>(host.data() + moff[i]);
-                for (int l = 0; l < 64; l++) {
-                    for (int st = 0; st < 16; st++) {
-                        const auto u = b.U[(l & 31) * 32 + 2 * st + (l >> 5)];
-                        fm->a[l][st] = float2{static_cast<float>(u.real()), static_cast<float>(u.imag())};
-                    }
-                }
-            } else if (b.U.size() >= 64) {
-                auto *m = reinterpret_cast<dcx<T> *>(host.data() + moff[i]);
-                for (size_t j = 0; j < b.U.size(); j++) {
-                    m[j] = {static_cast<T>(b.U[j].real()), static_cast<T>(b.U[j].imag())};
-                }
-            }
-        }
+        const size_t mbytes = kFuseSlots * slot;
+        std::vector<unsigned char> host(mbytes);
         unsigned char *dev = nullptr;
-        HIP_OK(hipMalloc(&dev, host.size()));
+        HIP_OK(hipMalloc(&dev, mbytes + ntiles * sizeof(double) + 256));
         struct Free {
             unsigned char *p;
             hipStream_t s;
@@ -332,7 +304,6 @@ template <class T> struct Engine {
                 (void)hipFree(p);
             }
         } guard{dev, h->stream};
-        HIP_OK(hipMemcpyAsync(dev, host.data(), mbytes, hipMemcpyHostToDevice, h->stream));
         auto *partial = reinterpret_cast<double *>(dev + mbytes);
         auto *scale = partial + ntiles;
         bool pending = false; // scale holds a factor not yet applied
@@ -342,47 +313,80 @@ template <class T> struct Engine {
             woff[i + 1] = woff[i] + t.nwires[i];
             poff[i + 1] = poff[i] + t.nparams[i];
         }
-        for (size_t i = 0; i < blocks.size(); i++) {
-            const auto &b = blocks[i];
+        size_t used_slots = 0;
+        int nb = 0;
+        FusedBlock b;
+        while (sched.next(b)) {
+            nb++;
             if (!b.fused) {
                 const int o = b.ops[0];
                 gate(sv, t.ids[o], t.wires + woff[o], t.nwires[o], t.inverse[o] != 0, t.params + poff[o],
                      t.nparams[o]);
-                continue;
-            }
-            if (mfma_ok(b, flags)) {
-                if constexpr (kMfma) {
-                    // persistent grid: the three workgroups a CU holds at once, contiguous tile ranges
-                    const uint64_t want = std::min<uint64_t>(ntiles, uint64_t(device_cus()) * 3);
-                    const uint64_t per = (ntiles + want - 1) / want;
-                    const uint64_t nblk = (ntiles + per - 1) / per; // no empty workgroup
-                    hipLaunchKernelGGL(k_fused5_mfma<kF5ColBits>, dim3(static_cast<unsigned>(nblk)),
-                                       dim3(2 << kF5ColBits), 0,
-                                       h->stream, sv, ntiles, per, fused5_geom(b.bits),
-                                       reinterpret_cast<const Fused5Mat *>(dev + moff[i]),
-                                       pending ? scale : nullptr, renorm ? partial : nullptr);
-                    HIP_OK(hipGetLastError());
-                    if (renorm) {
-                        hipLaunchKernelGGL(k_norm_scale, dim3(1), dim3(1024), 0, h->stream, partial, nblk,
-                                           scale);
+            } else {
+                const bool mf = mfma_ok(b, flags);
+                const bool big = !mf && b.U.size() >= 64; // matrix in device memory
+                unsigned char *hs = nullptr, *ds = nullptr;
+                if (mf || big) {
+                    if (used_slots == kFuseSlots) {
+                        HIP_OK(hipStreamSynchronize(h->stream));
+                        used_slots = 0;
+                    }
+                    hs = host.data() + used_slots * slot;
+                    ds = dev + used_slots * slot;
+                    used_slots++;
+                }
+                if (mf) {
+                    auto *fm = reinterpret_cast<Fused5Mat *>(hs);
+                    for (int l = 0; l < 64; l++) {
+                        for (int st = 0; st < 16; st++) {
+                            const auto u = b.U[(l & 31) * 32 + 2 * st + (l >> 5)];
+                            fm->a[l][st] = float2{static_cast<float>(u.real()), static_cast<float>(u.imag())};
+                        }
+                    }
+                    HIP_OK(hipMemcpyAsync(ds, hs, sizeof(Fused5Mat), hipMemcpyHostToDevice, h->stream));
+                    if constexpr (kMfma) {
+                        // persistent grid: the three workgroups a CU holds at once, contiguous tile ranges
+                        const uint64_t want = std::min<uint64_t>(ntiles, uint64_t(device_cus()) * 3);
+                        const uint64_t per = (ntiles + want - 1) / want;
+                        const uint64_t nblk = (ntiles + per - 1) / per; // no empty workgroup
+                        hipLaunchKernelGGL(k_fused5_mfma<kF5ColBits>, dim3(static_cast<unsigned>(nblk)),
+                                           dim3(2 << kF5ColBits), 0, h->stream, sv, ntiles, per,
+                                           fused5_geom(b.bits), reinterpret_cast<const Fused5Mat *>(ds),
+                                           pending ? scale : nullptr, renorm ? partial : nullptr);
                         HIP_OK(hipGetLastError());
-                        pending = true;
-                    } else {
-                        pending = false;
+                        if (renorm) {
+                            hipLaunchKernelGGL(k_norm_scale, dim3(1), dim3(1024), 0, h->stream, partial, nblk,
+                                               scale);
+                            HIP_OK(hipGetLastError());
+                            pending = true;
+                        } else {
+                            pending = false;
+                        }
+                    }
+                } else {
+                    std::vector<std::complex<T>> mt(b.U.size());
+                    for (size_t j = 0; j < b.U.size(); j++) {
+                        mt[j] = {static_cast<T>(b.U[j].real()), static_cast<T>(b.U[j].imag())};
+                    }
+                    if (big) {
+                        auto *m = reinterpret_cast<dcx<T> *>(hs);
+                        for (size_t j = 0; j < b.U.size(); j++) {
+                            m[j] = {mt[j].real(), mt[j].imag()};
+                        }
+                        HIP_OK(hipMemcpyAsync(ds, hs, b.U.size() * sizeof(dcx<T>), hipMemcpyHostToDevice,
+                                              h->stream));
+                    }
+                    const auto g = make_group<T>(b.bits, 0, block_offsets(b.bits), std::move(mt));
+                    switch (g.M) {
+                    case 8: launch_big_dev<8>(g, sv, reinterpret_cast<const dcx<T> *>(ds)); break;
+                    case 16: launch_big_dev<16>(g, sv, reinterpret_cast<const dcx<T> *>(ds)); break;
+                    case 32: launch_big_dev<32>(g, sv, reinterpret_cast<const dcx<T> *>(ds)); break;
+                    default: apply(g, sv); break; // M <= 4: matrix in the kernel arguments
                     }
                 }
-                continue;
             }
-            std::vector<std::complex<T>> mt(b.U.size());
-            for (size_t j = 0; j < b.U.size(); j++) {
-                mt[j] = {static_cast<T>(b.U[j].real()), static_cast<T>(b.U[j].imag())};
-            }
-            const auto g = make_group<T>(b.bits, 0, block_offsets(b.bits), std::move(mt));
-            switch (g.M) {
-            case 8: launch_big_dev<8>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
-            case 16: launch_big_dev<16>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
-            case 32: launch_big_dev<32>(g, sv, reinterpret_cast<const dcx<T> *>(dev + moff[i])); break;
-            default: apply(g, sv); break; // M <= 4: matrix in the kernel arguments
+            if (keep) {
+                keep->push_back(std::move(b));
             }
         }
         if (renorm) {
@@ -397,10 +401,6 @@ template <class T> struct Engine {
             HIP_OK(hipGetLastError());
         }
         HIP_OK(hipStreamSynchronize(h->stream));
-        const int nb = static_cast<int>(blocks.size());
-        if (keep) {
-            *keep = std::move(blocks);
-        }
         return nb;
     }
 
