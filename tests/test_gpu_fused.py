@@ -80,3 +80,23 @@ def test_fused_rejects_bad_width():
     with pytest.raises(ValueError):
         sv.apply_tape_fused(Tape().add("RX", [5], [0.1]), 5)
     sv.close()
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_fused_long_tape_reuses_matrix_slots(oracle, dtype):
+    """More blocks than the executor's 256 matrix slots: it waits for the
+    queued blocks and starts the slots over part-way through (both the
+    matrix-core blocks of complex64 and the dense-matrix blocks of
+    complex128)."""
+    from lightning_unification_amd import capi
+    n = 12
+    tape = random_layers(n, 330, seed=55)  # 2569 operations, 319 blocks
+    psi = random_state(n, 9)
+    exp = oracle.apply_tape(psi.copy(), tape)
+    print(f"fused long tape: {len(tape.ops)} ops {np.dtype(dtype).name}", flush=True)
+    with capi.DeviceStateVector.from_array(psi.astype(dtype)) as sv:
+        nb = sv.apply_tape_fused(tape, 5, renorm=dtype == np.complex64)
+        got = sv.state()
+    assert nb > 300
+    tol = 1e-12 if dtype == np.complex128 else 2e-5 * np.sqrt(nb)
+    assert np.max(np.abs(got - exp)) < tol * max(1.0, np.max(np.abs(exp)))
