@@ -207,18 +207,38 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_parity(v2<T> *const *__restr
     }
 }
 
-// jac[row + k] = factor * sum_b partial[b * nobs + k]: block k, 256 threads
-// each summing a strided share of the blocks, then a fixed-order block sum
-__global__ __launch_bounds__(kBlock) void k_adjoint_final(const double *__restrict__ partial, int nblocks,
-                                                          int nobs, double factor, double *__restrict__ jac) {
+// Trainable steps whose partial sums wait in their own slots of the partial
+// buffer, reduced together by one launch (k_adjoint_final_many).
+constexpr int kAdjFinalSlots = 64;
+struct AdjFinal {
+    double factor; // the generator's scale times -2 (sign of the inverse)
+    int nblocks;   // blocks of the step's launch, each wrote nobs partials
+    int nobs;      // observables of the launch
+    int dst;       // jacobian offset of its first observable
+};
+struct AdjFinals {
+    AdjFinal t[kAdjFinalSlots];
+};
+
+// For slot s = blockIdx.y and observable k = blockIdx.x:
+// jac[dst + k] = factor * sum_b partial_s[b * nobs + k], 256 threads each
+// summing a strided share of the blocks, then a fixed-order block sum.
+__global__ __launch_bounds__(kBlock) void k_adjoint_final_many(const double *__restrict__ partial,
+                                                               size_t slot_stride, const AdjFinals tab,
+                                                               double *__restrict__ jac) {
+    const AdjFinal t = tab.t[blockIdx.y];
     const int k = blockIdx.x;
+    if (k >= t.nobs) {
+        return; // the whole block
+    }
+    const double *p = partial + static_cast<size_t>(blockIdx.y) * slot_stride;
     double s = 0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
-        s += partial[static_cast<size_t>(b) * nobs + k];
+    for (int b = threadIdx.x; b < t.nblocks; b += kBlock) {
+        s += p[static_cast<size_t>(b) * t.nobs + k];
     }
     const double2 r = block_sum2(double2{s, 0});
     if (threadIdx.x == 0) {
-        jac[k] = factor * r.x;
+        jac[t.dst + k] = t.factor * r.x;
     }
 }
 

@@ -1326,7 +1326,10 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
     }
     HIP_OK(hipMalloc(&dstates, tables.size() * sizeof(V *)));
     HIP_OK(hipMalloc(&djac, size_t(ntp) * K * sizeof(double)));
-    HIP_OK(hipMalloc(&dpart, size_t(grid) * kAdjMaxObs * sizeof(double)));
+    // each trainable step's partial sums in a slot of their own; the slots'
+    // final sums run kAdjFinalSlots at a time (one launch, not one per step)
+    const size_t slot_stride = size_t(grid) * kAdjMaxObs;
+    HIP_OK(hipMalloc(&dpart, kAdjFinalSlots * slot_stride * sizeof(double)));
     struct Free {
         void *a, *b, *c;
         ~Free() {
@@ -1343,6 +1346,27 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
     for (auto &o : ops) {
         num_par_ops += o.p.empty() ? 0 : 1;
     }
+    AdjFinals pending{};
+    int npending = 0;
+    auto flush_finals = [&] {
+        if (npending > 0) {
+            hipLaunchKernelGGL(k_adjoint_final_many, dim3(kAdjMaxObs, npending), dim3(kBlock), 0, E.h->stream,
+                               dpart, slot_stride, pending, djac);
+            HIP_OK(hipGetLastError());
+            npending = 0;
+        }
+    };
+    // the slot for the next trainable step's partials (all slots in use: their
+    // sums first)
+    auto slot = [&] {
+        if (npending == kAdjFinalSlots) {
+            flush_finals();
+        }
+        return dpart + static_cast<size_t>(npending) * slot_stride;
+    };
+    auto finish = [&](unsigned nblocks, int nk, double factor, int dst) {
+        pending.t[npending++] = AdjFinal{factor, static_cast<int>(nblocks), nk, dst};
+    };
     ScratchScope sc2(h);
     V *mu = nullptr;
     size_t chunk_base = 0; // first pointer table of the current sweep
@@ -1435,7 +1459,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                     g1 = {g.parity_phase[1].real(), g.parity_phase[1].imag()};
                     hipLaunchKernelGGL((k_adjoint_parity<T, true>), dim3(grid), dim3(kBlock), 0,
                                        E.h->stream, st, nk, E.len(), ud.parity_mask, u0, u1, g0,
-                                       g1, upd, dpart);
+                                       g1, upd, slot());
                 } else {
                     hipLaunchKernelGGL((k_adjoint_parity<T, false>), dim3(grid), dim3(kBlock), 0,
                                        E.h->stream, st, nk, E.len(), ud.parity_mask, u0, u1, g0,
@@ -1462,7 +1486,7 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                     const unsigned gr = std::min<unsigned>(grid_for(nwork, 1, 8), kReduceBlocks);
                     if (train) {
                         hipLaunchKernelGGL((k_adjoint_step<T, M, true>), dim3(gr), dim3(kBlock), 0,
-                                           E.h->stream, st, nk, nwork, a, upd, dpart);
+                                           E.h->stream, st, nk, nwork, a, upd, slot());
                     } else {
                         hipLaunchKernelGGL((k_adjoint_step<T, M, false>), dim3(gr), dim3(kBlock), 0,
                                            E.h->stream, st, nk, nwork, a, upd, dpart);
@@ -1475,24 +1499,21 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                 case 2: gr = go(std::integral_constant<int, 2>{}); break;
                 default: gr = go(std::integral_constant<int, 4>{}); break;
                 }
-                if (train) {
-                    HIP_OK(hipGetLastError());
-                    hipLaunchKernelGGL(k_adjoint_final, dim3(nk), dim3(kBlock), 0, E.h->stream, dpart,
-                                       (int)gr, nk, factor, djac + row + k0);
-                }
                 HIP_OK(hipGetLastError());
+                if (train) {
+                    finish(gr, nk, factor, row + k0);
+                }
                 continue;
             }
             HIP_OK(hipGetLastError());
             if (train) {
-                hipLaunchKernelGGL(k_adjoint_final, dim3(nk), dim3(kBlock), 0, E.h->stream, dpart,
-                                   (int)grid, nk, factor, djac + row + k0);
-                HIP_OK(hipGetLastError());
+                finish(grid, nk, factor, row + k0);
             }
         }
     }
     chunk_base += static_cast<size_t>((ns + kAdjMaxObs - 1) / kAdjMaxObs);
     }
+    flush_finals();
     std::vector<double> J(size_t(ntp) * K);
     HIP_OK(hipMemcpyAsync(J.data(), djac, J.size() * sizeof(double), hipMemcpyDeviceToHost,
                           E.h->stream));
