@@ -45,14 +45,17 @@ struct Geom {
 };
 
 // Insert a zero bit at each pinned position (ascending) and set the fixed bits.
+// The loop leaves after the nb pinned positions (nb is uniform), so a
+// one-qubit gate pays for one insertion, not for twelve guarded ones.
 __device__ __forceinline__ uint64_t expand(uint64_t g, const Geom &geo) {
 #pragma unroll
     for (int j = 0; j < 12; j++) {
-        if (j < geo.nb) {
-            const int p = geo.pos[j];
-            const uint64_t lo = g & ((uint64_t{1} << p) - 1);
-            g = ((g >> p) << (p + 1)) | lo;
+        if (j >= geo.nb) {
+            break;
         }
+        const int p = geo.pos[j];
+        const uint64_t lo = g & ((uint64_t{1} << p) - 1);
+        g = ((g >> p) << (p + 1)) | lo;
     }
     return g | geo.fixed;
 }
