@@ -1483,7 +1483,9 @@ void adjoint_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, co
                     }
                     a.ud_diag = ud.diag ? 1 : 0;
                     const uint64_t nwork = E.len() >> ud.nb;
-                    const unsigned gr = std::min<unsigned>(grid_for(nwork, 1, 8), kReduceBlocks);
+                    // a trainable step ends with one cross-lane reduction per observable
+                    // in every wavefront: two work items per thread amortise it
+                    const unsigned gr = std::min<unsigned>(grid_for(nwork, train ? 2 : 1, 8), kReduceBlocks);
                     if (train) {
                         hipLaunchKernelGGL((k_adjoint_step<T, M, true>), dim3(gr), dim3(kBlock), 0,
                                            E.h->stream, st, nk, nwork, a, upd, slot());
