@@ -279,15 +279,29 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_sparse_moments(const v2<T> *__restrict__ sv, uint64_t len,
                                                            const int64_t *__restrict__ row_map,
                                                            const int64_t *__restrict__ cols,
-                                                           const dcx<T> *__restrict__ vals,
+                                                           const dcx<T> *__restrict__ vals, int64_t nnz,
+                                                           unsigned *__restrict__ bad,
                                                            double2 *__restrict__ partial) {
+    // The arrays are checked here, not on the host: a row whose range is not
+    // 0 <= row_map[i] <= row_map[i + 1] <= nnz sets bit 0 of *bad, a column
+    // index outside the state bit 1; neither is read through.
     double2 acc{0, 0};
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < len;
          i += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const int64_t lo = row_map[i], hi = row_map[i + 1];
+        if (lo < 0 || hi < lo || hi > nnz) {
+            atomicOr(bad, 1U);
+            continue;
+        }
         double yr = 0, yi = 0;
-        for (int64_t j = row_map[i]; j < row_map[i + 1]; j++) {
+        for (int64_t j = lo; j < hi; j++) {
+            const int64_t c = cols[j];
+            if (static_cast<uint64_t>(c) >= len) {
+                atomicOr(bad, 2U);
+                continue;
+            }
             const dcx<T> a = vals[j];
-            const v2<T> x = sv[cols[j]];
+            const v2<T> x = sv[c];
             yr += double(a.r) * double(x.x) - double(a.i) * double(x.y);
             yi += double(a.r) * double(x.y) + double(a.i) * double(x.x);
         }

@@ -2268,16 +2268,9 @@ static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_si
     if (row_map[0] != 0 || row_map[row_map_size - 1] != nnz) {
         throw InvalidArg("row_map must start at 0 and end at the number of non-zeros");
     }
-    for (int64_t i = 0; i + 1 < row_map_size; i++) {
-        if (row_map[i + 1] < row_map[i]) {
-            throw InvalidArg("row_map must be non-decreasing");
-        }
-    }
-    for (int64_t j = 0; j < nnz; j++) {
-        if (entries[j] < 0 || static_cast<uint64_t>(entries[j]) >= h->len) {
-            throw InvalidArg("column index out of range");
-        }
-    }
+    // the order of row_map and the range of every column index are checked
+    // by k_sparse_moments as it reads them (one pass over the arrays, on the
+    // device); the result is discarded and the call fails if either is wrong
     dispatch_or_throw(h, [&](auto &E) {
         using T = typename std::remove_reference_t<decltype(E)>::Scalar;
         // complex128 values are uploaded as given; complex64 ones are rounded first
@@ -2306,19 +2299,30 @@ static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_si
             HIP_OK(hipMemcpyAsync(dev + rb + eb, vsrc, nnz * sizeof(dcx<T>), hipMemcpyHostToDevice, h->stream));
         }
         const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 4), kReduceBlocks);
-        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + 1) * sizeof(double2)));
+        // partials, then the result, then the check flags (read back together)
+        auto *part = reinterpret_cast<double2 *>(reduce_buf(h, (kReduceBlocks + 2) * sizeof(double2)));
+        auto *bad = reinterpret_cast<unsigned *>(part + kReduceBlocks + 1);
+        HIP_OK(hipMemsetAsync(bad, 0, sizeof(unsigned), h->stream));
         hipLaunchKernelGGL((k_sparse_moments<T>), dim3(grid), dim3(kBlock), 0, h->stream, E.psi(), E.len(),
                            reinterpret_cast<const int64_t *>(dev), reinterpret_cast<const int64_t *>(dev + rb),
-                           reinterpret_cast<const dcx<T> *>(dev + rb + eb), part);
+                           reinterpret_cast<const dcx<T> *>(dev + rb + eb), nnz, bad, part);
         HIP_OK(hipGetLastError());
         hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kBlock), 0, h->stream, part, (int)grid,
                            part + kReduceBlocks);
         HIP_OK(hipGetLastError());
-        double2 r;
-        HIP_OK(hipMemcpyAsync(&r, part + kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        double2 r[2];
+        HIP_OK(hipMemcpyAsync(r, part + kReduceBlocks, sizeof(r), hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
-        *mean = r.x;
-        *mean_square = r.y;
+        unsigned flags = 0;
+        std::memcpy(&flags, &r[1], sizeof(flags));
+        if (flags & 1U) {
+            throw InvalidArg("row_map must be non-decreasing");
+        }
+        if (flags & 2U) {
+            throw InvalidArg("column index out of range");
+        }
+        *mean = r[0].x;
+        *mean_square = r[0].y;
     });
 }
 

@@ -57,6 +57,19 @@ def test_sparse_expval_var(oracle, dtype):
     bad[0] = 1 << n
     with pytest.raises(ValueError):
         sv.expval_sparse(rm, bad, va)
+    bad[0] = -1
+    with pytest.raises(ValueError):
+        sv.var_sparse(rm, bad, va)
+    # a row_map that decreases inside while its ends stay 0 and nnz: the
+    # device checks each row's range before reading through it
+    i = int(np.nonzero((rm[2:] > rm[1:-1]) & (rm[2:] < en.size))[0][0]) + 1
+    bad_rm = rm.copy()
+    bad_rm[i] = rm[i + 1] + 1
+    assert bad_rm[0] == 0 and bad_rm[-1] == en.size and np.any(np.diff(bad_rm) < 0)
+    with pytest.raises(ValueError, match="non-decreasing"):
+        sv.expval_sparse(bad_rm, en, va)
+    # the state and the handle are untouched by the refused calls
+    assert abs(sv.expval_sparse(rm, en, va) - e_exp) < tol * max(1, abs(e_exp))
     sv.close()
 
 
