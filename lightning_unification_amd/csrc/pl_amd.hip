@@ -1772,6 +1772,14 @@ int pl_sv_destroy(pl_sv_t h) {
             (void)hipFree(b.first);
         }
         (void)hipFree(h->d);
+        if (h->pin) {
+            (void)hipHostFree(h->pin);
+        }
+        for (hipEvent_t e : h->pin_ev) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
         for (hipEvent_t e : h->slots) {
             if (e) {
                 (void)hipEventDestroy(e);
@@ -2254,6 +2262,67 @@ int pl_vjp(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wi
     });
 }
 
+// Host-to-device copy of a caller's (pageable) array, queued on the handle's
+// stream.  Below kPinMin bytes it is one hipMemcpyAsync.  Above, chunks of
+// kPinSlot bytes alternate between the handle's two pinned slots: up to
+// kPinThreads host threads copy chunk c into one slot while the DMA engine
+// drains chunk c - 1 from the other, and a slot is refilled only after the
+// event recorded behind its last copy.  The slots (2 x 64 MiB) are allocated
+// on first use and kept until pl_sv_destroy.  The copies stay queued when
+// this returns: the caller keeps src alive until it synchronises the stream.
+constexpr size_t kPinSlot = size_t{64} << 20;
+constexpr size_t kPinMin = size_t{8} << 20;
+constexpr unsigned kPinThreads = 8;
+
+static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
+    if (bytes < kPinMin) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        return;
+    }
+    if (!h->pin) {
+        for (hipEvent_t &e : h->pin_ev) {
+            if (!e) {
+                HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+        }
+        HIP_OK(hipHostMalloc(&h->pin, 2 * kPinSlot, hipHostMallocDefault));
+    }
+    const unsigned hw = std::max(1U, std::thread::hardware_concurrency());
+    const unsigned nthr = std::min(kPinThreads, hw);
+    auto *out = static_cast<char *>(dst);
+    const auto *in = static_cast<const char *>(src);
+    for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
+        const size_t len = std::min(kPinSlot, bytes - off);
+        const int s = static_cast<int>(c & 1);
+        char *slot = static_cast<char *>(h->pin) + s * kPinSlot;
+        if (h->pin_busy[s]) {
+            HIP_OK(hipEventSynchronize(h->pin_ev[s])); // the slot's previous copy has left it
+            h->pin_busy[s] = false;
+        }
+        // page-aligned parts, one per thread (the caller's thread takes the first)
+        const size_t per = ((len + nthr - 1) / nthr + 4095) & ~size_t{4095};
+        std::vector<std::thread> th;
+        auto join_all = [&] {
+            for (auto &t : th) {
+                t.join();
+            }
+        };
+        try {
+            for (size_t p = per; p < len; p += per) {
+                th.emplace_back([=] { std::memcpy(slot + p, in + off + p, std::min(per, len - p)); });
+            }
+        } catch (...) {
+            join_all(); // a thread that could not start: finish the ones that did, then fail
+            throw;
+        }
+        std::memcpy(slot, in + off, std::min(per, len));
+        join_all();
+        HIP_OK(hipMemcpyAsync(out + off, slot, len, hipMemcpyHostToDevice, h->stream));
+        HIP_OK(hipEventRecord(h->pin_ev[s], h->stream));
+        h->pin_busy[s] = true;
+    }
+}
+
 // Measurements::expval / var of a CSR matrix (MeasurementsLQubit.hpp:183-214,
 // 437-471; SparseLinAlg.hpp apply_Sparse_Matrix): both from one fused pass.
 static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_size,
@@ -2291,12 +2360,10 @@ static void sparse_moments(pl_sv_t h, const int64_t *row_map, int64_t row_map_si
             unsigned char *p;
             ~Free() { (void)hipFree(p); }
         } fr{dev};
-        HIP_OK(hipMemcpyAsync(dev, row_map, rb, hipMemcpyHostToDevice, h->stream));
+        upload_host(h, dev, row_map, rb);
         if (nnz > 0) {
-            HIP_OK(hipMemcpyAsync(dev + rb, entries, nnz * 8, hipMemcpyHostToDevice, h->stream));
-        }
-        if (nnz > 0) {
-            HIP_OK(hipMemcpyAsync(dev + rb + eb, vsrc, nnz * sizeof(dcx<T>), hipMemcpyHostToDevice, h->stream));
+            upload_host(h, dev + rb, entries, nnz * 8);
+            upload_host(h, dev + rb + eb, vsrc, nnz * sizeof(dcx<T>));
         }
         const unsigned grid = std::min<unsigned>(grid_for(E.len(), 1, 4), kReduceBlocks);
         // partials, then the result, then the check flags (read back together)

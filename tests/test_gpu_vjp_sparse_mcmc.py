@@ -73,6 +73,36 @@ def test_sparse_expval_var(oracle, dtype):
     sv.close()
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_sparse_large_arrays_in_chunks(oracle, dtype):
+    """Arrays of 8 MiB or more are copied through the handle's two pinned
+    64 MiB slots in chunks.  About 6.3e6 non-zeros at 20 qubits: the column
+    indices fill one chunk and the complex128 values two, the second of which
+    reuses the slot the indices left; a second call reuses both slots."""
+    import scipy.sparse as sp
+    from lightning_unification_amd import capi
+    n = 20
+    N = 1 << n
+    rs = np.random.default_rng(12)
+    k = 3
+    a = sp.csr_matrix((rs.normal(size=N * k) + 1j * rs.normal(size=N * k),
+                       (np.repeat(np.arange(N), k), rs.integers(0, N, size=N * k))), shape=(N, N))
+    H = (a + a.getH()).tocsr()
+    H.sort_indices()
+    rm, en, va = H.indptr.astype(np.int64), H.indices.astype(np.int64), H.data.astype(np.complex128)
+    assert en.nbytes > 8 << 20 and va.nbytes > 64 << 20
+    s = random_state(n, 31, dtype)
+    s128 = s.astype(np.complex128)
+    e_exp = oracle.sparse(s128, rm, en, va)
+    v_exp = oracle.sparse(s128, rm, en, va, var=True)
+    tol = 1e-12 if dtype == np.complex128 else 1e-5
+    print(f"sparse chunks {np.dtype(dtype).name} nnz {en.size}", flush=True)
+    with capi.DeviceStateVector.from_array(s) as sv:
+        for _ in range(2):
+            assert abs(sv.expval_sparse(rm, en, va) - e_exp) < tol * max(1, abs(e_exp))
+            assert abs(sv.var_sparse(rm, en, va) - v_exp) < tol * max(1, abs(v_exp))
+
+
 def _tv(samples, p):
     n = samples.shape[1]
     idx = samples.astype(np.int64) @ (1 << np.arange(n - 1, -1, -1))
