@@ -309,9 +309,12 @@ def bench_next():
     - <H> and Var(H) of a seeded sparse Hermitian H (pl_expval_sparse /
       pl_var_sparse) at 22 qubits, 5 non-zeros per row (the diagonal and four
       bit-flip partners).  The boundary takes host CSR arrays, as the
-      reference's does, so the time includes the host checks and the PCIe
-      upload of the 24 B per non-zero; the kernel alone reads those bytes
-      plus psi once.
+      reference's does, so the time includes the PCIe upload of the 24 B per
+      non-zero; the kernel alone reads those bytes plus psi once and checks
+      the indices as it reads them.
+    - Whole-state transfers (pl_sv_upload / pl_sv_download) of a 28-qubit
+      complex128 state (4 GiB) from and to pageable host memory: the
+      PCIe-inclusive rate of the boundary's copies, never the headline.
     - Metropolis samples (pl_generate_mcmc_samples), 10^5 shots after 100
       burn-in steps, with both transition kernels."""
     from lightning_unification_amd import capi
@@ -372,8 +375,32 @@ def bench_next():
     out["sparse_expval_var"] = {"qubits": ns, "nnz": int(entries.size), "expval": e, "var": v,
                                 "expval_ms": round(t_e * 1e3, 2), "var_ms": round(t_v * 1e3, 2),
                                 "csr_GB": round(csr_bytes / 1e9, 3),
-                                "includes": "host checks and PCIe upload of the CSR arrays"}
+                                "includes": "PCIe upload of the CSR arrays"}
     log(f"sparse H at {ns} qubits, nnz {entries.size}: <H> {t_e*1e3:.1f} ms, var {t_v*1e3:.1f} ms")
+    svs.close()
+    del row_map, entries, values
+
+    nt = 28
+    host = np.zeros(1 << nt, np.complex128)
+    host.real[::4097] = 1.0
+    with capi.DeviceStateVector(nt, np.complex128) as svt:
+        svt.upload(host)  # warm: first use of the pinned slots
+        t0 = time.perf_counter()
+        for _ in range(2):
+            svt.upload(host)
+        t_up = (time.perf_counter() - t0) / 2
+        back = svt.state()
+        t0 = time.perf_counter()
+        for _ in range(2):
+            back = svt.state()
+        t_down = (time.perf_counter() - t0) / 2
+        assert np.array_equal(back, host)
+    out["state_transfer"] = {"qubits": nt, "GB": round(host.nbytes / 1e9, 3),
+                             "upload_ms": round(t_up * 1e3, 1), "download_ms": round(t_down * 1e3, 1),
+                             "upload_GBs": round(host.nbytes / t_up / 1e9, 1),
+                             "download_GBs": round(host.nbytes / t_down / 1e9, 1),
+                             "host": "pageable"}
+    log(f"state transfer, {nt} qubits: upload {t_up*1e3:.0f} ms, download {t_down*1e3:.0f} ms")
     return out
 
 

@@ -226,12 +226,6 @@ struct pl_sv {
     size_t staging_bytes = 0;
     std::vector<hipEvent_t> slots; // pl_event_record slots
     std::vector<std::pair<void *, size_t>> bufs; // grow-only device buffers by purpose (plamd::Buf)
-    // two pinned host slots for large uploads of caller arrays (upload_host);
-    // pin_ev[s] is recorded after the last copy queued out of slot s, and
-    // pin_busy[s] says that such a copy exists (wait on the event before reuse)
-    void *pin = nullptr;
-    hipEvent_t pin_ev[2] = {nullptr, nullptr};
-    bool pin_busy[2] = {false, false};
     // gates queued while deferral is on (pl_sv_set_deferred), in the tape
     // encoding of pl_sv_apply_ops; applied as tile passes before anything
     // else reads or changes the state

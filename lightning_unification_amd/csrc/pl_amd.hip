@@ -1772,14 +1772,6 @@ int pl_sv_destroy(pl_sv_t h) {
             (void)hipFree(b.first);
         }
         (void)hipFree(h->d);
-        if (h->pin) {
-            (void)hipHostFree(h->pin);
-        }
-        for (hipEvent_t e : h->pin_ev) {
-            if (e) {
-                (void)hipEventDestroy(e);
-            }
-        }
         for (hipEvent_t e : h->slots) {
             if (e) {
                 (void)hipEventDestroy(e);
@@ -1808,6 +1800,131 @@ int pl_sv_dtype(pl_sv_t h, int *out) {
     });
 }
 
+// Copies between a caller's (pageable) host array and device memory, on the
+// handle's stream.  Below kPinMin bytes each is one hipMemcpyAsync.  Above,
+// chunks of kPinSlot bytes alternate between two pinned slots: up to
+// kPinThreads host threads copy one chunk between the caller's array and a
+// slot while the DMA engine moves the neighbouring chunk through the other
+// slot.  The slots (2 x 64 MiB) belong to the process: allocated on first
+// use, kept until it ends, and taken by one transfer at a time (every handle
+// is on device 0).
+constexpr size_t kPinSlot = size_t{64} << 20;
+constexpr size_t kPinMin = size_t{8} << 20;
+constexpr unsigned kPinThreads = 8;
+
+struct PinRing {
+    std::mutex mu;       // held for a whole transfer
+    void *base = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr}; // recorded after the last copy queued through a slot
+    bool busy[2] = {false, false};         // such a copy exists: wait on ev before reuse
+};
+static PinRing g_pin;
+
+static char *pin_slot(int s) {
+    if (!g_pin.base) {
+        for (hipEvent_t &e : g_pin.ev) {
+            if (!e) {
+                HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+        }
+        HIP_OK(hipHostMalloc(&g_pin.base, 2 * kPinSlot, hipHostMallocDefault));
+    }
+    return static_cast<char *>(g_pin.base) + s * kPinSlot;
+}
+
+// Waits until no queued copy reads or writes slot s.
+static void pin_wait(int s) {
+    if (g_pin.busy[s]) {
+        HIP_OK(hipEventSynchronize(g_pin.ev[s]));
+        g_pin.busy[s] = false;
+    }
+}
+
+// Queues a copy through slot s on h's stream and marks the slot busy behind it.
+static void pin_copy(pl_sv_t h, int s, void *dst, const void *src, size_t len, hipMemcpyKind kind) {
+    HIP_OK(hipMemcpyAsync(dst, src, len, kind, h->stream));
+    HIP_OK(hipEventRecord(g_pin.ev[s], h->stream));
+    g_pin.busy[s] = true;
+}
+
+// memcpy of len bytes split into page-aligned parts, one per thread (the
+// caller's thread takes the first)
+static void parallel_memcpy(char *dst, const char *src, size_t len) {
+    const unsigned nthr = std::min(kPinThreads, std::max(1U, std::thread::hardware_concurrency()));
+    const size_t per = ((len + nthr - 1) / nthr + 4095) & ~size_t{4095};
+    std::vector<std::thread> th;
+    auto join_all = [&] {
+        for (auto &t : th) {
+            t.join();
+        }
+    };
+    try {
+        for (size_t p = per; p < len; p += per) {
+            th.emplace_back([=] { std::memcpy(dst + p, src + p, std::min(per, len - p)); });
+        }
+    } catch (...) {
+        join_all(); // a thread that could not start: finish the ones that did, then fail
+        throw;
+    }
+    std::memcpy(dst, src, std::min(per, len));
+    join_all();
+}
+
+// Host to device: chunk c is copied into slot c % 2 once that slot's previous
+// copy has left it.  The copies stay queued when this returns; the caller
+// keeps src alive until it synchronises the stream.
+static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
+    if (bytes < kPinMin) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pin.mu);
+    auto *out = static_cast<char *>(dst);
+    const auto *in = static_cast<const char *>(src);
+    for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
+        const size_t len = std::min(kPinSlot, bytes - off);
+        const int s = static_cast<int>(c & 1);
+        char *slot = pin_slot(s);
+        pin_wait(s);
+        parallel_memcpy(slot, in + off, len);
+        pin_copy(h, s, out + off, slot, len, hipMemcpyHostToDevice);
+    }
+}
+
+// Device to host, after the work queued before it: the DMA of chunk c into
+// slot c % 2 is queued before chunk c - 1 is copied out of the other slot.
+// Returns when every byte is in dst.
+static void download_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
+    if (bytes < kPinMin) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pin.mu);
+    auto *out = static_cast<char *>(dst);
+    const auto *in = static_cast<const char *>(src);
+    const size_t nchunks = (bytes + kPinSlot - 1) / kPinSlot;
+    auto drain = [&](size_t c) { // chunk c from its slot to dst
+        const int s = static_cast<int>(c & 1);
+        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
+        pin_wait(s);
+        parallel_memcpy(out + off, pin_slot(s), len);
+    };
+    for (size_t c = 0; c < nchunks; c++) {
+        const int s = static_cast<int>(c & 1);
+        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
+        char *slot = pin_slot(s);
+        // slot s last held chunk c - 2, copied out by the drain below; the wait
+        // covers a copy through it from an earlier transfer, on any handle's stream
+        pin_wait(s);
+        pin_copy(h, s, slot, in + off, len, hipMemcpyDeviceToHost);
+        if (c > 0) {
+            drain(c - 1);
+        }
+    }
+    drain(nchunks - 1);
+}
+
 int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {
     return guarded([&] {
         if (!h || !host) throw InvalidArg("null argument");
@@ -1815,7 +1932,7 @@ int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {
             throw InvalidArg("upload size does not match the state vector size");
         }
         discard_deferred(h);
-        HIP_OK(hipMemcpyAsync(h->d, host, nbytes, hipMemcpyHostToDevice, h->stream));
+        upload_host(h, h->d, host, nbytes);
         HIP_OK(hipStreamSynchronize(h->stream));
         g_h2d_bytes.fetch_add(nbytes, std::memory_order_relaxed);
     });
@@ -1827,8 +1944,7 @@ int pl_sv_download(pl_sv_t h, void *host, size_t nbytes) {
             throw InvalidArg("download size does not match the state vector size");
         }
         flush_deferred(h);
-        HIP_OK(hipMemcpyAsync(host, h->d, nbytes, hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipStreamSynchronize(h->stream));
+        download_host(h, host, h->d, nbytes);
         g_d2h_bytes.fetch_add(nbytes, std::memory_order_relaxed);
     });
 }
@@ -2260,67 +2376,6 @@ int pl_vjp(pl_sv_t h, int nops, const int *ids, const int *nwires, const int *wi
         }
         vjp_sweep(E, nops, ids, nwires, wires, inverse, nparams, params, dy, ntp, tp, vjp);
     });
-}
-
-// Host-to-device copy of a caller's (pageable) array, queued on the handle's
-// stream.  Below kPinMin bytes it is one hipMemcpyAsync.  Above, chunks of
-// kPinSlot bytes alternate between the handle's two pinned slots: up to
-// kPinThreads host threads copy chunk c into one slot while the DMA engine
-// drains chunk c - 1 from the other, and a slot is refilled only after the
-// event recorded behind its last copy.  The slots (2 x 64 MiB) are allocated
-// on first use and kept until pl_sv_destroy.  The copies stay queued when
-// this returns: the caller keeps src alive until it synchronises the stream.
-constexpr size_t kPinSlot = size_t{64} << 20;
-constexpr size_t kPinMin = size_t{8} << 20;
-constexpr unsigned kPinThreads = 8;
-
-static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
-    if (bytes < kPinMin) {
-        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
-        return;
-    }
-    if (!h->pin) {
-        for (hipEvent_t &e : h->pin_ev) {
-            if (!e) {
-                HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            }
-        }
-        HIP_OK(hipHostMalloc(&h->pin, 2 * kPinSlot, hipHostMallocDefault));
-    }
-    const unsigned hw = std::max(1U, std::thread::hardware_concurrency());
-    const unsigned nthr = std::min(kPinThreads, hw);
-    auto *out = static_cast<char *>(dst);
-    const auto *in = static_cast<const char *>(src);
-    for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
-        const size_t len = std::min(kPinSlot, bytes - off);
-        const int s = static_cast<int>(c & 1);
-        char *slot = static_cast<char *>(h->pin) + s * kPinSlot;
-        if (h->pin_busy[s]) {
-            HIP_OK(hipEventSynchronize(h->pin_ev[s])); // the slot's previous copy has left it
-            h->pin_busy[s] = false;
-        }
-        // page-aligned parts, one per thread (the caller's thread takes the first)
-        const size_t per = ((len + nthr - 1) / nthr + 4095) & ~size_t{4095};
-        std::vector<std::thread> th;
-        auto join_all = [&] {
-            for (auto &t : th) {
-                t.join();
-            }
-        };
-        try {
-            for (size_t p = per; p < len; p += per) {
-                th.emplace_back([=] { std::memcpy(slot + p, in + off + p, std::min(per, len - p)); });
-            }
-        } catch (...) {
-            join_all(); // a thread that could not start: finish the ones that did, then fail
-            throw;
-        }
-        std::memcpy(slot, in + off, std::min(per, len));
-        join_all();
-        HIP_OK(hipMemcpyAsync(out + off, slot, len, hipMemcpyHostToDevice, h->stream));
-        HIP_OK(hipEventRecord(h->pin_ev[s], h->stream));
-        h->pin_busy[s] = true;
-    }
 }
 
 // Measurements::expval / var of a CSR matrix (MeasurementsLQubit.hpp:183-214,
