@@ -71,9 +71,14 @@ int pl_sv_num_qubits(pl_sv_t sv, int *out);
 int pl_sv_dtype(pl_sv_t sv, int *out);
 /* Host -> device copy of 2^n amplitudes (StateVectorLQubitRaw(ptr, len) /
  * updateData(ptr, len), StateVectorLQubitRaw.hpp:78-87,
- * StateVectorLQubitManaged.hpp:166-169).  nbytes must equal the state size. */
+ * StateVectorLQubitManaged.hpp:166-169).  nbytes must equal the state size.
+ * host may be pageable memory.  From 8 MiB on, this call, pl_sv_download and
+ * the sparse measurements copy through two pinned host slots of 64 MiB that
+ * the library allocates at the first such copy and holds until the process
+ * ends; concurrent copies through them take turns. */
 int pl_sv_upload(pl_sv_t sv, const void *host, size_t nbytes);
-/* Device -> host copy (getDataVector / reading the numpy buffer). */
+/* Device -> host copy (getDataVector / reading the numpy buffer), after the
+ * work queued on sv; returns with every byte in host. */
 int pl_sv_download(pl_sv_t sv, void *host, size_t nbytes);
 /* dst <- src (copy constructor, StateVectorLQubitManaged.hpp:89-94).  The
  * copy is queued on dst's stream after src's queued work, and src's stream
@@ -187,7 +192,10 @@ int pl_generate_samples(pl_sv_t sv, uint64_t seed, int64_t nsamples,
  * (MeasurementsLQubit.hpp:183-214, 437-471; SparseLinAlg.hpp): row_map has
  * 2^n + 1 entries (row_map[i] = non-zeros before row i), entries and values
  * (interleaved complex) have nnz each.  One fused pass computes <psi|H psi>
- * and ||H psi||^2 with FP64 accumulation; var = ||H psi||^2 - <psi|H psi>^2. */
+ * and ||H psi||^2 with FP64 accumulation; var = ||H psi||^2 - <psi|H psi>^2.
+ * row_map[0] = 0 and row_map[2^n] = nnz are checked on the host; the order of
+ * row_map and the range of every column index are checked by the pass, which
+ * reads through neither, and the call then fails with PL_ERR_INVALID. */
 int pl_expval_sparse(pl_sv_t sv, const int64_t *row_map, int64_t row_map_size,
                      const int64_t *entries, const double *values, int64_t nnz,
                      double *out);
