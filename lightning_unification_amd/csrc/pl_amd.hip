@@ -19,6 +19,131 @@
 
 namespace plamd {
 
+// Copies between a caller's (pageable) host array and device memory, on the
+// handle's stream.  Below kPinMin bytes each is one hipMemcpyAsync.  Above,
+// chunks of kPinSlot bytes alternate between two pinned slots: up to
+// kPinThreads host threads copy one chunk between the caller's array and a
+// slot while the DMA engine moves the neighbouring chunk through the other
+// slot.  The slots (2 x 64 MiB) belong to the process: allocated on first
+// use, kept until it ends, and taken by one transfer at a time (every handle
+// is on device 0).
+constexpr size_t kPinSlot = size_t{64} << 20;
+constexpr size_t kPinMin = size_t{8} << 20;
+constexpr unsigned kPinThreads = 8;
+
+struct PinRing {
+    std::mutex mu;       // held for a whole transfer
+    void *base = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr}; // recorded after the last copy queued through a slot
+    bool busy[2] = {false, false};         // such a copy exists: wait on ev before reuse
+};
+static PinRing g_pin;
+
+static char *pin_slot(int s) {
+    if (!g_pin.base) {
+        for (hipEvent_t &e : g_pin.ev) {
+            if (!e) {
+                HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            }
+        }
+        HIP_OK(hipHostMalloc(&g_pin.base, 2 * kPinSlot, hipHostMallocDefault));
+    }
+    return static_cast<char *>(g_pin.base) + s * kPinSlot;
+}
+
+// Waits until no queued copy reads or writes slot s.
+static void pin_wait(int s) {
+    if (g_pin.busy[s]) {
+        HIP_OK(hipEventSynchronize(g_pin.ev[s]));
+        g_pin.busy[s] = false;
+    }
+}
+
+// Queues a copy through slot s on h's stream and marks the slot busy behind it.
+static void pin_copy(pl_sv_t h, int s, void *dst, const void *src, size_t len, hipMemcpyKind kind) {
+    HIP_OK(hipMemcpyAsync(dst, src, len, kind, h->stream));
+    HIP_OK(hipEventRecord(g_pin.ev[s], h->stream));
+    g_pin.busy[s] = true;
+}
+
+// memcpy of len bytes split into page-aligned parts, one per thread (the
+// caller's thread takes the first)
+static void parallel_memcpy(char *dst, const char *src, size_t len) {
+    const unsigned nthr = std::min(kPinThreads, std::max(1U, std::thread::hardware_concurrency()));
+    const size_t per = ((len + nthr - 1) / nthr + 4095) & ~size_t{4095};
+    std::vector<std::thread> th;
+    auto join_all = [&] {
+        for (auto &t : th) {
+            t.join();
+        }
+    };
+    try {
+        for (size_t p = per; p < len; p += per) {
+            th.emplace_back([=] { std::memcpy(dst + p, src + p, std::min(per, len - p)); });
+        }
+    } catch (...) {
+        join_all(); // a thread that could not start: finish the ones that did, then fail
+        throw;
+    }
+    std::memcpy(dst, src, std::min(per, len));
+    join_all();
+}
+
+// Host to device: chunk c is copied into slot c % 2 once that slot's previous
+// copy has left it.  The copies stay queued when this returns; the caller
+// keeps src alive until it synchronises the stream.
+static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
+    if (bytes < kPinMin) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pin.mu);
+    auto *out = static_cast<char *>(dst);
+    const auto *in = static_cast<const char *>(src);
+    for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
+        const size_t len = std::min(kPinSlot, bytes - off);
+        const int s = static_cast<int>(c & 1);
+        char *slot = pin_slot(s);
+        pin_wait(s);
+        parallel_memcpy(slot, in + off, len);
+        pin_copy(h, s, out + off, slot, len, hipMemcpyHostToDevice);
+    }
+}
+
+// Device to host, after the work queued before it: the DMA of chunk c into
+// slot c % 2 is queued before chunk c - 1 is copied out of the other slot.
+// Returns when every byte is in dst.
+static void download_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
+    if (bytes < kPinMin) {
+        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_pin.mu);
+    auto *out = static_cast<char *>(dst);
+    const auto *in = static_cast<const char *>(src);
+    const size_t nchunks = (bytes + kPinSlot - 1) / kPinSlot;
+    auto drain = [&](size_t c) { // chunk c from its slot to dst
+        const int s = static_cast<int>(c & 1);
+        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
+        pin_wait(s);
+        parallel_memcpy(out + off, pin_slot(s), len);
+    };
+    for (size_t c = 0; c < nchunks; c++) {
+        const int s = static_cast<int>(c & 1);
+        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
+        char *slot = pin_slot(s);
+        // slot s last held chunk c - 2, copied out by the drain below; the wait
+        // covers a copy through it from an earlier transfer, on any handle's stream
+        pin_wait(s);
+        pin_copy(h, s, slot, in + off, len, hipMemcpyDeviceToHost);
+        if (c > 0) {
+            drain(c - 1);
+        }
+    }
+    drain(nchunks - 1);
+}
+
 template <class T> struct Engine {
     using V = v2<T>;
     using Scalar = T;
@@ -906,8 +1031,7 @@ template <class T> struct Engine {
             hipLaunchKernelGGL((k_probs<T>), dim3(grid_for(len())), dim3(kBlock), 0, h->stream,
                                psi(), len(), d);
             HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(out, d, len() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-            HIP_OK(hipStreamSynchronize(h->stream));
+            download_host(h, out, d, len() * sizeof(double));
             return;
         }
         if (nw > n() || nw > 32) {
@@ -970,8 +1094,7 @@ template <class T> struct Engine {
                                dim3(kBlock), 0, h->stream, psi(), nbins, nrest, m, d);
         }
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(out, d, nbins * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipStreamSynchronize(h->stream));
+        download_host(h, out, d, nbins * sizeof(double));
     }
 
     // Exclusive-prefix tables of |psi|^2 for inverse-CDF sampling
@@ -1016,7 +1139,7 @@ template <class T> struct Engine {
         auto *du = static_cast<double *>(buf(h, kBufUniforms, m * sizeof(double)));
         auto *didx = static_cast<int64_t *>(buf(h, kBufIndices, m * sizeof(int64_t)));
         if (host_u) {
-            HIP_OK(hipMemcpyAsync(du, host_u, m * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            upload_host(h, du, host_u, m * sizeof(double));
         } else {
             hipLaunchKernelGGL(k_uniforms, dim3((m + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                h->stream, seed, m, du);
@@ -1024,8 +1147,7 @@ template <class T> struct Engine {
         }
         sample_device(du, m, didx);
         if (host_idx) {
-            HIP_OK(hipMemcpyAsync(host_idx, didx, m * sizeof(int64_t), hipMemcpyDeviceToHost,
-                                  h->stream));
+            download_host(h, host_idx, didx, m * sizeof(int64_t));
         }
         if (host_bits) {
             const uint64_t tot = uint64_t(m) * n();
@@ -1033,8 +1155,7 @@ template <class T> struct Engine {
             hipLaunchKernelGGL(k_sample_bits, dim3((tot + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                h->stream, didx, m, n(), dbits);
             HIP_OK(hipGetLastError());
-            HIP_OK(hipMemcpyAsync(host_bits, dbits, tot * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                  h->stream));
+            download_host(h, host_bits, dbits, tot * sizeof(uint64_t));
         }
         HIP_OK(hipStreamSynchronize(h->stream));
     }
@@ -1108,8 +1229,7 @@ template <class T> struct Engine {
         hipLaunchKernelGGL(k_sample_bits, dim3((tot + kBlock - 1) / kBlock), dim3(kBlock), 0, h->stream, didx, m,
                            n(), dbits);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(host_bits, dbits, tot * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipStreamSynchronize(h->stream));
+        download_host(h, host_bits, dbits, tot * sizeof(uint64_t));
     }
 
     // Measurements::expval(Observable) and var(Observable)
@@ -1577,7 +1697,7 @@ void vjp_sweep(Engine<T> &E, int nops, const int *ids, const int *nwires, const 
             host[i].x = static_cast<T>(dy[2 * i]);
             host[i].y = static_cast<T>(dy[2 * i + 1]);
         }
-        HIP_OK(hipMemcpyAsync(mu, host.data(), E.len() * sizeof(V), hipMemcpyHostToDevice, h->stream));
+        upload_host(h, mu, host.data(), E.len() * sizeof(V));
         HIP_OK(hipStreamSynchronize(h->stream));
     }
     long long cur_param = static_cast<long long>(num_par_ops) - 1;
@@ -1798,131 +1918,6 @@ int pl_sv_dtype(pl_sv_t h, int *out) {
         if (!h || !out) throw InvalidArg("null argument");
         *out = h->dtype;
     });
-}
-
-// Copies between a caller's (pageable) host array and device memory, on the
-// handle's stream.  Below kPinMin bytes each is one hipMemcpyAsync.  Above,
-// chunks of kPinSlot bytes alternate between two pinned slots: up to
-// kPinThreads host threads copy one chunk between the caller's array and a
-// slot while the DMA engine moves the neighbouring chunk through the other
-// slot.  The slots (2 x 64 MiB) belong to the process: allocated on first
-// use, kept until it ends, and taken by one transfer at a time (every handle
-// is on device 0).
-constexpr size_t kPinSlot = size_t{64} << 20;
-constexpr size_t kPinMin = size_t{8} << 20;
-constexpr unsigned kPinThreads = 8;
-
-struct PinRing {
-    std::mutex mu;       // held for a whole transfer
-    void *base = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr}; // recorded after the last copy queued through a slot
-    bool busy[2] = {false, false};         // such a copy exists: wait on ev before reuse
-};
-static PinRing g_pin;
-
-static char *pin_slot(int s) {
-    if (!g_pin.base) {
-        for (hipEvent_t &e : g_pin.ev) {
-            if (!e) {
-                HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            }
-        }
-        HIP_OK(hipHostMalloc(&g_pin.base, 2 * kPinSlot, hipHostMallocDefault));
-    }
-    return static_cast<char *>(g_pin.base) + s * kPinSlot;
-}
-
-// Waits until no queued copy reads or writes slot s.
-static void pin_wait(int s) {
-    if (g_pin.busy[s]) {
-        HIP_OK(hipEventSynchronize(g_pin.ev[s]));
-        g_pin.busy[s] = false;
-    }
-}
-
-// Queues a copy through slot s on h's stream and marks the slot busy behind it.
-static void pin_copy(pl_sv_t h, int s, void *dst, const void *src, size_t len, hipMemcpyKind kind) {
-    HIP_OK(hipMemcpyAsync(dst, src, len, kind, h->stream));
-    HIP_OK(hipEventRecord(g_pin.ev[s], h->stream));
-    g_pin.busy[s] = true;
-}
-
-// memcpy of len bytes split into page-aligned parts, one per thread (the
-// caller's thread takes the first)
-static void parallel_memcpy(char *dst, const char *src, size_t len) {
-    const unsigned nthr = std::min(kPinThreads, std::max(1U, std::thread::hardware_concurrency()));
-    const size_t per = ((len + nthr - 1) / nthr + 4095) & ~size_t{4095};
-    std::vector<std::thread> th;
-    auto join_all = [&] {
-        for (auto &t : th) {
-            t.join();
-        }
-    };
-    try {
-        for (size_t p = per; p < len; p += per) {
-            th.emplace_back([=] { std::memcpy(dst + p, src + p, std::min(per, len - p)); });
-        }
-    } catch (...) {
-        join_all(); // a thread that could not start: finish the ones that did, then fail
-        throw;
-    }
-    std::memcpy(dst, src, std::min(per, len));
-    join_all();
-}
-
-// Host to device: chunk c is copied into slot c % 2 once that slot's previous
-// copy has left it.  The copies stay queued when this returns; the caller
-// keeps src alive until it synchronises the stream.
-static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
-    if (bytes < kPinMin) {
-        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
-        return;
-    }
-    std::lock_guard<std::mutex> lock(g_pin.mu);
-    auto *out = static_cast<char *>(dst);
-    const auto *in = static_cast<const char *>(src);
-    for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
-        const size_t len = std::min(kPinSlot, bytes - off);
-        const int s = static_cast<int>(c & 1);
-        char *slot = pin_slot(s);
-        pin_wait(s);
-        parallel_memcpy(slot, in + off, len);
-        pin_copy(h, s, out + off, slot, len, hipMemcpyHostToDevice);
-    }
-}
-
-// Device to host, after the work queued before it: the DMA of chunk c into
-// slot c % 2 is queued before chunk c - 1 is copied out of the other slot.
-// Returns when every byte is in dst.
-static void download_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
-    if (bytes < kPinMin) {
-        HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
-        HIP_OK(hipStreamSynchronize(h->stream));
-        return;
-    }
-    std::lock_guard<std::mutex> lock(g_pin.mu);
-    auto *out = static_cast<char *>(dst);
-    const auto *in = static_cast<const char *>(src);
-    const size_t nchunks = (bytes + kPinSlot - 1) / kPinSlot;
-    auto drain = [&](size_t c) { // chunk c from its slot to dst
-        const int s = static_cast<int>(c & 1);
-        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
-        pin_wait(s);
-        parallel_memcpy(out + off, pin_slot(s), len);
-    };
-    for (size_t c = 0; c < nchunks; c++) {
-        const int s = static_cast<int>(c & 1);
-        const size_t off = c * kPinSlot, len = std::min(kPinSlot, bytes - off);
-        char *slot = pin_slot(s);
-        // slot s last held chunk c - 2, copied out by the drain below; the wait
-        // covers a copy through it from an earlier transfer, on any handle's stream
-        pin_wait(s);
-        pin_copy(h, s, slot, in + off, len, hipMemcpyDeviceToHost);
-        if (c > 0) {
-            drain(c - 1);
-        }
-    }
-    drain(nchunks - 1);
 }
 
 int pl_sv_upload(pl_sv_t h, const void *host, size_t nbytes) {

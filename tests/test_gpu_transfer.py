@@ -56,3 +56,31 @@ def test_two_handles_share_the_slots():
             b.upload(a0 if k % 2 else b0)
             assert np.array_equal(a.state(), b0 if k % 2 else a0)
             assert np.array_equal(b.state(), a0 if k % 2 else b0)
+
+
+def test_large_outputs_and_inputs(oracle):
+    """The measurements' large host arrays take the same chunked path:
+    probabilities of every basis state (16 MiB at 21 qubits, 32 MiB at 22),
+    marginal probabilities of 21 of 22 wires (16 MiB), 10^6 uniforms in and
+    indices out (8 MB each), and 2 x 10^5 bit rows of 22 qubits (35 MB)."""
+    capi = _capi()
+    for n in (21, 22):
+        psi = random_state(n, 40 + n)
+        print(f"transfer: probs n={n}", flush=True)
+        with capi.DeviceStateVector.from_array(psi) as sv:
+            assert rel_err(sv.probs(), oracle.probs(psi)) < 1e-12
+            if n == 22:
+                w = list(range(1, n))
+                assert rel_err(sv.probs(w), oracle.probs(psi, w)) < 1e-12
+                u = np.sort(np.random.default_rng(9).uniform(size=1_000_000))
+                print("transfer: 10^6 samples", flush=True)
+                got = sv.sample(u)
+                assert np.array_equal(got, oracle.sample_cdf(psi, u))
+                bits = sv.generate_samples(200_000, seed=11)
+                assert bits.shape == (200_000, n) and bits.nbytes > 8 << 20
+                # sample s depends on (seed, s) alone: the first rows of the
+                # chunked copy equal a call small enough for a plain copy
+                head = sv.generate_samples(1000, seed=11)
+                assert head.nbytes < 8 << 20
+                assert np.array_equal(bits[:1000], head)
+                assert np.all((bits == 0) | (bits == 1))
