@@ -72,10 +72,11 @@ int pl_sv_dtype(pl_sv_t sv, int *out);
 /* Host -> device copy of 2^n amplitudes (StateVectorLQubitRaw(ptr, len) /
  * updateData(ptr, len), StateVectorLQubitRaw.hpp:78-87,
  * StateVectorLQubitManaged.hpp:166-169).  nbytes must equal the state size.
- * host may be pageable memory.  From 8 MiB on, this call, pl_sv_download and
- * the sparse measurements copy through two pinned host slots of 64 MiB that
- * the library allocates at the first such copy and holds until the process
- * ends; concurrent copies through them take turns. */
+ * host may be pageable memory.  Every host array of 8 MiB or more that a call
+ * reads or fills (states, probabilities, samples, uniforms, the VJP's
+ * cotangent, sparse matrices) is copied through two pinned host slots of
+ * 64 MiB that the library allocates at the first such copy and holds until
+ * the process ends; concurrent copies through them take turns. */
 int pl_sv_upload(pl_sv_t sv, const void *host, size_t nbytes);
 /* Device -> host copy (getDataVector / reading the numpy buffer), after the
  * work queued on sv; returns with every byte in host. */
