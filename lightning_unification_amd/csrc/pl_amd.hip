@@ -26,7 +26,7 @@ namespace plamd {
 // slot while the DMA engine moves the neighbouring chunk through the other
 // slot.  The slots (2 x 64 MiB) belong to the process: allocated on first
 // use, kept until it ends, and taken by one transfer at a time (every handle
-// is on device 0).
+// is on device 0).  Where they cannot be pinned every copy is a plain one.
 constexpr size_t kPinSlot = size_t{64} << 20;
 constexpr size_t kPinMin = size_t{8} << 20;
 constexpr unsigned kPinThreads = 8;
@@ -34,20 +34,31 @@ constexpr unsigned kPinThreads = 8;
 struct PinRing {
     std::mutex mu;       // held for a whole transfer
     void *base = nullptr;
+    bool unavailable = false; // pinning was refused once: plain copies from then on
     hipEvent_t ev[2] = {nullptr, nullptr}; // recorded after the last copy queued through a slot
     bool busy[2] = {false, false};         // such a copy exists: wait on ev before reuse
 };
 static PinRing g_pin;
 
-static char *pin_slot(int s) {
-    if (!g_pin.base) {
+// Allocates the slots on first use; false if the process cannot pin them
+// (a locked-memory limit, say), and the transfers then use plain copies.
+static bool pin_ready() {
+    if (!g_pin.base && !g_pin.unavailable) {
         for (hipEvent_t &e : g_pin.ev) {
             if (!e) {
                 HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
             }
         }
-        HIP_OK(hipHostMalloc(&g_pin.base, 2 * kPinSlot, hipHostMallocDefault));
+        if (hipHostMalloc(&g_pin.base, 2 * kPinSlot, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError(); // the refusal is not left for the next call to report
+            g_pin.base = nullptr;
+            g_pin.unavailable = true;
+        }
     }
+    return g_pin.base != nullptr;
+}
+
+static char *pin_slot(int s) {
     return static_cast<char *>(g_pin.base) + s * kPinSlot;
 }
 
@@ -93,11 +104,14 @@ static void parallel_memcpy(char *dst, const char *src, size_t len) {
 // copy has left it.  The copies stay queued when this returns; the caller
 // keeps src alive until it synchronises the stream.
 static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
-    if (bytes < kPinMin) {
+    std::unique_lock<std::mutex> lock(g_pin.mu, std::defer_lock);
+    if (bytes >= kPinMin) {
+        lock.lock();
+    }
+    if (bytes < kPinMin || !pin_ready()) {
         HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
         return;
     }
-    std::lock_guard<std::mutex> lock(g_pin.mu);
     auto *out = static_cast<char *>(dst);
     const auto *in = static_cast<const char *>(src);
     for (size_t off = 0, c = 0; off < bytes; off += kPinSlot, c++) {
@@ -114,12 +128,15 @@ static void upload_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
 // slot c % 2 is queued before chunk c - 1 is copied out of the other slot.
 // Returns when every byte is in dst.
 static void download_host(pl_sv_t h, void *dst, const void *src, size_t bytes) {
-    if (bytes < kPinMin) {
+    std::unique_lock<std::mutex> lock(g_pin.mu, std::defer_lock);
+    if (bytes >= kPinMin) {
+        lock.lock();
+    }
+    if (bytes < kPinMin || !pin_ready()) {
         HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         return;
     }
-    std::lock_guard<std::mutex> lock(g_pin.mu);
     auto *out = static_cast<char *>(dst);
     const auto *in = static_cast<const char *>(src);
     const size_t nchunks = (bytes + kPinSlot - 1) / kPinSlot;
