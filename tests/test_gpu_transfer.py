@@ -84,3 +84,29 @@ def test_large_outputs_and_inputs(oracle):
                 assert head.nbytes < 8 << 20
                 assert np.array_equal(bits[:1000], head)
                 assert np.all((bits == 0) | (bits == 1))
+
+
+def test_module_mirror_of_a_large_buffer(oracle):
+    """The pybind module's device mirror of a 20-qubit (16 MiB) numpy buffer,
+    driven as the device does: the first wrap uploads through the pinned
+    slots, the gates run on the device, and the write-back at the wrapper's
+    release fills the (write-watched) buffer through them.  A host write
+    afterwards is seen, and the next wrap uploads it again."""
+    from lightning_unification_amd import lightning_qubit_ops as ops
+    from lightning_unification_amd.circuits import Tape, plain_layers
+    from test_gpu_pybind import apply_lightning
+    n = 20
+    tape, _, _ = plain_layers(n, 1, seed=8)
+    state = random_state(n, 9)
+    buf = state.copy()
+    print("transfer: module mirror, n=20", flush=True)
+    up0, down0 = ops.transfer_counters()
+    apply_lightning(ops, buf, tape)
+    assert rel_err(buf, oracle.apply_tape(state.copy(), tape)) < 1e-12
+    up1, down1 = ops.transfer_counters()
+    assert (up1 - up0, down1 - down0) == (buf.nbytes, buf.nbytes)
+    buf[:] = state  # a host write into the mirrored buffer
+    apply_lightning(ops, buf, Tape().add("Hadamard", [0]))
+    assert rel_err(buf, oracle.apply_tape(state.copy(), Tape().add("Hadamard", [0]))) < 1e-12
+    up2, _ = ops.transfer_counters()
+    assert up2 - up1 == buf.nbytes
