@@ -467,13 +467,14 @@ def distinct_xmasks(ham, n):
 
 
 def bench_measurements(sv, ham, uniforms, n, reps=3):
-    """<H> of the 100-term Hamiltonian and inverse-CDF sampling of the
-    uniforms on the state in sv: one warm call each, then `reps` calls timed
+    """<H> of the 100-term Hamiltonian, inverse-CDF sampling of the
+    uniforms and the marginal probabilities of 8 wires on the state in sv:
+    one warm call each, then `reps` calls timed
     with HIP events on the engine's stream (the best is reported) and by the
     wall clock (host checks, table staging and the copy of the indices
     back included).  Algorithmic bytes: one read of the state per distinct
     x-mask for <H>; one read of the state plus 16 bytes per sample for the
-    sampler."""
+    sampler; one read of the state for the probabilities."""
     S = (1 << n) * 16
     out = {}
     e = sv.expval(ham)
@@ -511,6 +512,22 @@ def bench_measurements(sv, ham, uniforms, n, reps=3):
                       "first_indices": [int(i) for i in idx[:4]]}
     log(f"<H> = {e:.12f}: {out['expval_100_terms']['ms']} ms ({out['expval_100_terms']['GBs']} GB/s); "
         f"{m} samples: {out['samples']['ms']} ms ({out['samples']['GBs']} GB/s)")
+    # marginal probabilities of up to 8 scattered wires (pl_probs): one read of the
+    # state, 2^8 bins back to the host
+    wires = [w for w in (0, 3, 7, 11, 15, 19, 23) if w < n - 1] + [n - 1]
+    p = sv.probs(wires)
+    dev = []
+    for _ in range(reps):
+        sv.synchronize()
+        sv.event_record(0)
+        p = sv.probs(wires)
+        sv.event_record(1)
+        dev.append(sv.event_elapsed(0, 1))
+    ms = min(dev)
+    out["probs_8_wires"] = {"wires": wires, "ms": round(ms, 3), "sum": float(p.sum()),
+                            "GBs": round(S / (ms / 1e3) / 1e9, 1),
+                            "hbm_frac": round(S / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"probs of {len(wires)} wires: {out['probs_8_wires']['ms']} ms ({out['probs_8_wires']['GBs']} GB/s)")
     return out
 
 
