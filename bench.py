@@ -914,6 +914,7 @@ def main():
         "per_gate_GBs": per_gate_gbs,
         "expval_100_terms": meas["expval_100_terms"],
         "samples": meas["samples"],
+        "probs_8_wires": meas["probs_8_wires"],
         "tiled_circuit": tiled,
         "c1_adjoint": c1,
         "next_rows": next_rows,
