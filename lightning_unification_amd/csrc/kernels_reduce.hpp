@@ -238,6 +238,54 @@ __global__ __launch_bounds__(kBlock) void k_marginal(const v2<T> *__restrict__ s
     }
 }
 
+// Marginal probabilities when L of the marginal bits lie below bit 3 (inside
+// one 128-byte line of complex128 amplitudes).  One bin per block would read
+// only part of every line; here blockIdx.y fixes the pattern of the other
+// marginal bits (deposited into himask) and each thread reads all 2^L
+// amplitudes of a line that differ in the low marginal bits (lomask),
+// keeping one sum per pattern.  The bin of a bit pattern follows bitpos, as
+// bin_bits does in reverse.  Partials are laid out as k_marginal's.
+__device__ __forceinline__ uint64_t pattern_bin(uint64_t bits, const MarginalGeom &m) {
+    uint64_t bin = 0;
+    for (int j = 0; j < m.nw; j++) {
+        bin |= ((bits >> m.bitpos[j]) & 1U) << (m.nw - 1 - j);
+    }
+    return bin;
+}
+
+template <class T, int L>
+__global__ __launch_bounds__(kBlock) void k_marginal_lo(const v2<T> *__restrict__ sv, uint64_t nrest,
+                                                        MarginalGeom m, uint64_t lomask, uint64_t himask,
+                                                        double *__restrict__ partial) {
+    constexpr int C = 1 << L;
+    static_assert(L >= 1 && L <= 3, "low marginal bits lie in bits 0..2");
+    uint64_t off[C];
+    double acc[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        off[c] = deposit(static_cast<uint64_t>(c), lomask);
+        acc[c] = 0;
+    }
+    const uint64_t hb = deposit(blockIdx.y, himask);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; r < nrest; r += stride) {
+        const uint64_t base = expand(r, m.rest) | hb;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const v2<T> a = sv[base | off[c]];
+            acc[c] += double(a.x * a.x + a.y * a.y);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < C; c += 2) {
+        const double2 s = block_sum2(double2{acc[c], acc[c + 1]});
+        if (threadIdx.x == 0) {
+            partial[pattern_bin(hb | off[c], m) * gridDim.x + blockIdx.x] = s.x;
+            partial[pattern_bin(hb | off[c + 1], m) * gridDim.x + blockIdx.x] = s.y;
+        }
+    }
+}
+
 // Final per-bin sum over `chunks` partials (thread per bin).
 static __global__ __launch_bounds__(kBlock) void k_marginal_final(const double *__restrict__ partial,
                                                            uint64_t nbins, int chunks,

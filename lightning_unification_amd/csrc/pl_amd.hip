@@ -1097,12 +1097,22 @@ template <class T> struct Engine {
             for (int j = 0; j < nw; j++) {
                 m.rest.pos[j] = bits[j];
             }
+            // marginal bits inside a 128-byte line: one block per pattern of
+            // the others, every thread reading whole lines (k_marginal_lo)
+            const uint64_t lomask = pinned & 7U, himask = pinned & ~uint64_t{7};
+            const int lo = __builtin_popcountll(lomask);
+            const uint64_t nys = nbins >> lo; // blocks along y
             const unsigned chunks = static_cast<unsigned>(std::max<uint64_t>(
                 1, std::min<uint64_t>((nrest + kBlock * 64 - 1) / (kBlock * 64),
-                                      std::max<uint64_t>(1, uint64_t(device_cus()) * 16 / nbins))));
+                                      std::max<uint64_t>(1, uint64_t(device_cus()) * 16 / nys))));
             double *part = reduce_buf(h, nbins * chunks * sizeof(double));
-            hipLaunchKernelGGL((k_marginal<T>), dim3(chunks, static_cast<unsigned>(nbins)),
-                               dim3(kBlock), 0, h->stream, psi(), nrest, m, part);
+            const dim3 grid(chunks, static_cast<unsigned>(nys));
+            switch (lo) {
+            case 0: hipLaunchKernelGGL((k_marginal<T>), grid, dim3(kBlock), 0, h->stream, psi(), nrest, m, part); break;
+            case 1: hipLaunchKernelGGL((k_marginal_lo<T, 1>), grid, dim3(kBlock), 0, h->stream, psi(), nrest, m, lomask, himask, part); break;
+            case 2: hipLaunchKernelGGL((k_marginal_lo<T, 2>), grid, dim3(kBlock), 0, h->stream, psi(), nrest, m, lomask, himask, part); break;
+            default: hipLaunchKernelGGL((k_marginal_lo<T, 3>), grid, dim3(kBlock), 0, h->stream, psi(), nrest, m, lomask, himask, part); break;
+            }
             HIP_OK(hipGetLastError());
             hipLaunchKernelGGL(k_marginal_final, dim3((nbins + kBlock - 1) / kBlock), dim3(kBlock), 0,
                                h->stream, part, nbins, (int)chunks, d);

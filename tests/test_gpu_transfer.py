@@ -110,3 +110,20 @@ def test_module_mirror_of_a_large_buffer(oracle):
     assert rel_err(buf, oracle.apply_tape(state.copy(), Tape().add("Hadamard", [0]))) < 1e-12
     up2, _ = ops.transfer_counters()
     assert up2 - up1 == buf.nbytes
+
+
+@pytest.mark.parametrize("wires", [[0, 5, 2], [17, 3, 9], [16, 17, 1], [15, 17, 0, 16],
+                                   [2, 17, 15, 7, 16, 0]])
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_marginals_with_low_wires(oracle, wires, dtype):
+    """Marginal probabilities whose wires include 0 to 3 of the three lowest
+    bits (wires n-1, n-2, n-3: the amplitudes of one 128-byte line), wires in
+    any order, at 18 qubits (2^12 or more amplitudes per bin)."""
+    capi = _capi()
+    n = 18
+    psi = random_state(n, 60 + len(wires), dtype)
+    print(f"marginal: wires {wires} {np.dtype(dtype).name}", flush=True)
+    with capi.DeviceStateVector.from_array(psi) as sv:
+        got = sv.probs(wires)
+    exp = oracle.probs(psi.astype(np.complex128), wires)
+    assert rel_err(got, exp) < (1e-12 if dtype == np.complex128 else 1e-6)
